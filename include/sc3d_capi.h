@@ -1,0 +1,152 @@
+/*
+ * sc3d_capi.h -- C ABI of libsc3d_hip.so, the MI355X (gfx950) implementation
+ * of Synergia's 3D open-boundary Hockney space-charge step.
+ *
+ * No C++ or torch types cross this boundary: plain structs, device pointers
+ * and sizes.  The C++ operator class synergia2_amd/host/
+ * space_charge_3d_open_hockney.h is the only in-tree caller; every other
+ * binding (ctypes in synergia2_amd/__init__.py) uses
+ * exactly these entry points.
+ *
+ * Reference interfaces replaced (paths relative to the synergia2 tree):
+ *   sc3d_options   <- Space_charge_3d_open_hockney_options,
+ *                     src/synergia/simulation/implemented_collective_options.h:67-126
+ *   sc3d_bunch     <- the Bunch accessors the operator reads,
+ *                     src/synergia/bunch/bunch.h:267-289,317,458-472 and the
+ *                     7-column LayoutLeft particle view, bunch_particles.h:24-43
+ *   sc3d_create    <- Space_charge_3d_open_hockney::Space_charge_3d_open_hockney
+ *                     + construct_workspaces,
+ *                     src/synergia/collective/space_charge_3d_open_hockney.cc:376-397,455-491
+ *   sc3d_apply     <- Space_charge_3d_open_hockney::apply_bunch,
+ *                     space_charge_3d_open_hockney.cc:422-453 (called per bunch by
+ *                     apply_impl, :399-420, behind Operator::apply, operator.h:62-65)
+ *   sc3d_sync      <- the Kokkos::fence() that ends every stage of apply_bunch
+ *                     (space_charge_3d_open_hockney.cc:613,638,650,653,671,675,749,783)
+ *   sc3d_last_error<- the std::runtime_error messages thrown by the operator
+ *                     (space_charge_3d_open_hockney.cc:512-516,594-598,619-623)
+ *   sc3d_debug_get <- no single counterpart: copies out the workspaces the
+ *                     reference keeps as members (rho2, phi2, enx/eny/enz,
+ *                     space_charge_3d_open_hockney.h:22-37) for parity tests.
+ *   sc3d_populate_gaussian <- populate_6d, src/synergia/bunch/populate.cc:128-145
+ *                     (own counter-based generator, not GSL ranlxd2)
+ */
+#ifndef SC3D_CAPI_H
+#define SC3D_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define SC3D_OK 0
+#define SC3D_ERR_ARG (-1)         /* invalid argument / unsupported shape    */
+#define SC3D_ERR_HIP (-2)         /* HIP runtime failure                      */
+#define SC3D_ERR_DOMAIN (-3)      /* all three extents negligible            */
+#define SC3D_ERR_PERIODIC (-4)    /* periodic_z not implemented (reference)  */
+#define SC3D_ERR_SIZE (-5)        /* workspace/bunch size limit exceeded     */
+
+#define SC3D_GREEN_POINTLIKE 0
+#define SC3D_GREEN_LINEAR 1
+
+#define SC3D_PRECISION_FP64 0     /* plain form: everything FP64             */
+#define SC3D_PRECISION_FP32_GRID 1 /* deposit/FFT/field FP32, particles FP64 */
+
+typedef struct sc3d_options {
+    /* the reference option fields, same meaning and defaults */
+    int32_t shape[3];            /* x, y, z; default 32, 32, 64              */
+    double size[3];              /* fixed-domain size  (domain_fixed only)   */
+    double offset[3];            /* fixed-domain centre (domain_fixed only)  */
+    int32_t green_fn;            /* default SC3D_GREEN_LINEAR                */
+    int32_t periodic_z;          /* default 0; 1 -> SC3D_ERR_PERIODIC        */
+    int32_t grid_entire_period;  /* default 0                                */
+    int32_t domain_fixed;        /* default 0                                */
+    int32_t comm_group_size;     /* default 4; accepted, one GPU per bunch   */
+    double z_period;             /* default 0                                */
+    double n_sigma;              /* default 8                                */
+    double kick_scale;           /* default 1                                */
+    /* build extensions (no reference counterpart) */
+    int32_t precision;           /* SC3D_PRECISION_*; default FP64           */
+    int32_t sort_period;         /* sort + reorder the bunch by cell every N
+                                    applies (0 = never); default 1           */
+} sc3d_options;
+
+typedef struct sc3d_bunch {
+    double* parts;               /* device; column j at parts + j*stride     */
+    int64_t stride;              /* capacity (LayoutLeft leading dimension)  */
+    int64_t n_active;            /* Bunch::size()                            */
+    uint8_t* masks;              /* device, one byte per slot, 1 = active    */
+    double total_num;            /* Bunch::get_total_num()                   */
+    double real_num;             /* Bunch::get_real_num()                    */
+    int32_t charge;              /* Bunch::get_particle_charge()             */
+    double mass;                 /* Bunch::get_mass() [GeV/c^2]              */
+    double ref_gamma, ref_beta, ref_p;  /* reference particle, p in GeV/c    */
+} sc3d_bunch;
+
+typedef struct sc3d_ctx sc3d_ctx;
+
+/* Fills the reference defaults for a gx x gy x gz grid
+ * (implemented_collective_options.h:82-95). */
+void sc3d_options_default(sc3d_options* opt, int32_t gx, int32_t gy, int32_t gz);
+
+/* Creates a context on the current HIP device: validates the options and
+ * allocates the grid workspaces. Shapes must be even, each doubled length a
+ * power of two in [8, 512]. */
+int sc3d_create(const sc3d_options* opt, sc3d_ctx** out);
+
+/* Enqueues one space-charge kick of length time_step [s] on `stream`
+ * (hipStream_t, NULL = default stream).  Asynchronous: errors found on the
+ * device (negligible extent) are reported by sc3d_sync.  When sort_period
+ * says so, the first n_active slots of the bunch (all 7 columns and the
+ * masks) are reordered in place by cell; particle identity is column 6. */
+int sc3d_apply(sc3d_ctx* ctx, const sc3d_bunch* bunch, double time_step,
+               void* stream);
+
+/* Waits for the context's last stream and reports device-side errors. */
+int sc3d_sync(sc3d_ctx* ctx);
+
+int sc3d_destroy(sc3d_ctx* ctx);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* sc3d_last_error(void);
+
+/* Debug taps: synchronise and copy a workspace to host memory. */
+#define SC3D_DBG_DOMAIN 0        /* sc3d_domain_record                       */
+#define SC3D_DBG_KEYS 1          /* int32[n] cell keys of the last sort, in  */
+                                 /* the bunch order before that sort         */
+#define SC3D_DBG_PERM 2          /* int32[n] stable sort permutation         */
+#define SC3D_DBG_RHO 3           /* double[G] charge density octant (zyx)    */
+#define SC3D_DBG_PHI 4           /* double[G] potential octant, unnormalised */
+#define SC3D_DBG_EX 5            /* double[G]                                */
+#define SC3D_DBG_EY 6
+#define SC3D_DBG_EZ 7
+#define SC3D_DBG_GHAT 8          /* double[(gz+1)(gy+1)(gx+1)] real Green's  */
+                                 /* spectrum, kz<=gz, ky<=gy, kx<=gx         */
+int sc3d_debug_get(sc3d_ctx* ctx, int what, void* host_dst, int64_t bytes);
+
+/* Domain record copied by SC3D_DBG_DOMAIN (all FP64). */
+typedef struct sc3d_domain_record {
+    double sums[6];              /* Sx Sy Scdt Sxx Syy Scdtcdt               */
+    double mean[3], stddev[3];
+    double size[3], offset[3];
+    double left[3], h[3];
+    double w0, fn_norm, factor;
+    double inv_h[3], half_inv_h[3];
+    double ghat_h[3];            /* cell size the cached spectrum is for     */
+    int32_t error, ghat_dirty, ghat_valid, pad;
+} sc3d_domain_record;
+
+/* Fills slots [0, n) of a bunch: coordinates = mean + L * u with u a
+ * standard normal 6-vector from Philox-4x32-10 keyed (seed, id) and
+ * Box-Muller, L lower-triangular packed row-major (21 entries); column 6 =
+ * id0 + slot; masks = 1.  Deterministic for a given (seed, id). */
+int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n,
+                           uint8_t* masks, const double mean[6],
+                           const double chol[21], uint64_t seed, int64_t id0,
+                           void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,837 @@
+// sc3d_oracle.cc -- CPU restatement of the reference 3D open-Hockney step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity checker of the HIP
+// library (tests/) and the timed CPU baseline (bench.py cpu_baseline leg).
+// The product never links, loads or calls it.
+//
+// Built by oracle/Makefile with -O2 -fopenmp -ffp-contract=off
+// -fno-fast-math so that every floating-point operation is the plain IEEE
+// operation written here (no FMA contraction), which is what makes the cell
+// indices reproducible bit for bit.
+//
+// What pins each function (tests/test_oracle_*.py):
+//   orc_tree_sums        -- independent numpy restatement of the fixed tree;
+//                           math.fsum within 1e-15; exact on +-a pairs.
+//   orc_compute_domain   -- closed form on symmetric fixtures; throw on
+//                           negligible extent; hand-computed constants.
+//   orc_cell_index/keys  -- integer closed forms on hand-built coordinates.
+//   orc_stable_sort      -- numpy.argsort(kind="stable").
+//   orc_fft1d/3d         -- numpy.fft golden vectors, roundtrip, Parseval.
+//   orc_green_fn2        -- independent numpy evaluation of the closed form,
+//                           mirror symmetry, far field -> hz/r.
+//   orc_deposit          -- single-particle closed form, charge
+//                           conservation, brute-force O(8N) numpy deposit.
+//   convolution (apply)  -- brute-force direct sum on a tiny grid.
+//   orc_force            -- linear/quadratic phi gives exact differences.
+//   orc_kick             -- per-particle closed form; rod-of-charge physics.
+#include "sc3d_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+// physical_constants.h:17-24 (PDG 2012) and :94-106
+constexpr double kPi = 3.141592653589793;  // Kokkos::numbers::pi_v<double>
+constexpr double kE = 1.602176565e-19;
+constexpr double kC = 299792458.0;
+constexpr double kMu0 = 4 * kPi * 1.0e-7;
+constexpr double kEpsilon0 = 1.0 / (kC * kC * kMu0);
+
+thread_local std::string g_err;
+
+int padded_real(int s) { return 2 * (s / 2 + 1); }  // distributed_fft3d.h:18-22
+int padded_cplx(int s) { return s / 2 + 1; }        // distributed_fft3d.h:24-28
+
+// space_charge_3d_open_hockney.cc:94-109
+double smallest_non_tiny(double val, double o1, double o2, double tiny)
+{
+    if (val > tiny) return val;
+    if ((o1 > tiny) && (o2 > tiny)) return std::min(o1, o2);
+    return std::max(o1, o2);
+}
+
+double wtime() { return omp_get_wtime(); }
+
+}  // namespace
+
+extern "C" const char* orc_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Bunch-extent sums (core_diagnostics.cc:139-153).  The reference's Kokkos
+// parallel_reduce order is unspecified; this build fixes it to one tree,
+// shared bit for bit with the device kernel:
+//   chunk c covers particles [4096c, 4096c+4096);
+//   lane t of the chunk adds, starting from +0.0 and in order j = 0..15, the
+//   values of particle 4096c + 256j + t (skipped when masked or >= n);
+//   the 256 lane sums are combined pairwise: s[t] = s[t] + s[t+w] for
+//   w = 1,2,...,128 and t a multiple of 2w; the chunk result is s[0];
+//   the chunk results P[c] are combined by the same lane/pairwise rule.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kTreeLanes = 256;
+constexpr int kTreeItems = 16;
+constexpr int64_t kTreeChunk = kTreeLanes * kTreeItems;
+
+void pairwise256(double (*s)[6])
+{
+    for (int w = 1; w < kTreeLanes; w *= 2)
+        for (int t = 0; t < kTreeLanes; t += 2 * w)
+            for (int q = 0; q < 6; ++q) s[t][q] = s[t][q] + s[t + w][q];
+}
+}  // namespace
+
+extern "C" void orc_tree_sums(const double* parts, int64_t stride, int64_t n,
+                              const uint8_t* masks, double out[6])
+{
+    int64_t nch = (n + kTreeChunk - 1) / kTreeChunk;
+    std::vector<double> P(static_cast<size_t>(nch) * 6);
+    static thread_local double s[kTreeLanes][6];
+    for (int64_t c = 0; c < nch; ++c) {
+        for (int t = 0; t < kTreeLanes; ++t) {
+            for (int q = 0; q < 6; ++q) s[t][q] = 0.0;
+            for (int j = 0; j < kTreeItems; ++j) {
+                int64_t i = c * kTreeChunk + int64_t(j) * kTreeLanes + t;
+                if (i >= n || !masks[i]) continue;
+                double x = parts[i], y = parts[i + 2 * stride],
+                       z = parts[i + 4 * stride];
+                s[t][0] += x;
+                s[t][1] += y;
+                s[t][2] += z;
+                s[t][3] += x * x;
+                s[t][4] += y * y;
+                s[t][5] += z * z;
+            }
+        }
+        pairwise256(s);
+        for (int q = 0; q < 6; ++q) P[c * 6 + q] = s[0][q];
+    }
+    for (int t = 0; t < kTreeLanes; ++t) {
+        for (int q = 0; q < 6; ++q) s[t][q] = 0.0;
+        for (int64_t c = t; c < nch; c += kTreeLanes)
+            for (int q = 0; q < 6; ++q) s[t][q] += P[c * 6 + q];
+    }
+    pairwise256(s);
+    for (int q = 0; q < 6; ++q) out[q] = s[0][q];
+}
+
+// ---------------------------------------------------------------------------
+// Domain (space_charge_3d_open_hockney.cc:376-397, 493-540;
+// core_diagnostics.cc:454-459; rectangular_grid_domain.h:20-36) and the
+// scalars derived from it: deposit weight0 (deposit.cc:868-870),
+// get_normalization_force (.cc:707-735), kick factor (.cc:759-770).
+// ---------------------------------------------------------------------------
+extern "C" int orc_compute_domain(const orc_options* opt, const orc_bunch* b,
+                                  double time_step, orc_domain* d)
+{
+    std::memset(d, 0, sizeof(*d));
+    double size[3], offset[3];
+    if (opt->domain_fixed) {
+        for (int a = 0; a < 3; ++a) {
+            size[a] = opt->size[a];
+            offset[a] = opt->offset[a];
+        }
+    } else {
+        orc_tree_sums(b->parts, b->stride, b->n_active, b->masks, d->sums);
+        double total = b->total_num;
+        for (int j = 0; j < 3; ++j) {
+            d->mean[j] = d->sums[j] / total;
+            d->stddev[j] =
+                std::sqrt(d->sums[j + 3] / total - d->mean[j] * d->mean[j]);
+        }
+        const double tiny = 1.0e-10;
+        if ((d->stddev[0] < tiny) && (d->stddev[1] < tiny) &&
+            (d->stddev[2] < tiny)) {
+            g_err = "Space_charge_3d_open_hockney_eigen::update_domain: "
+                    "all three spatial dimensions have neglible extent";
+            return -3;
+        }
+        offset[0] = d->mean[0];
+        offset[1] = d->mean[1];
+        offset[2] = d->mean[2];
+        size[0] = opt->n_sigma * smallest_non_tiny(d->stddev[0], d->stddev[1],
+                                                   d->stddev[2], tiny);
+        size[1] = opt->n_sigma * smallest_non_tiny(d->stddev[1], d->stddev[0],
+                                                   d->stddev[2], tiny);
+        size[2] = opt->n_sigma * smallest_non_tiny(d->stddev[2], d->stddev[0],
+                                                   d->stddev[1], tiny);
+        if (opt->grid_entire_period) {
+            offset[2] = 0.0;
+            size[2] = opt->z_period;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        d->size[a] = size[a];
+        d->offset[a] = offset[a];
+        d->left[a] = offset[a] - size[a] / 2.0;
+        d->h[a] = size[a] / (1.0 * opt->shape[a]);
+    }
+    const double* h = d->h;
+    d->w0 = (b->real_num / b->total_num) * b->charge * kE / (h[0] * h[1] * h[2]);
+
+    double norm = h[0] * h[1] * h[2];
+    norm *= 1.0 / (4.0 * kPi * kEpsilon0);
+    norm *= 1.0;
+    if (opt->green_fn == 1) norm *= 1.0 / (h[2] * h[2]);
+    norm *= 1.0 / (2 * opt->shape[0] * 2 * opt->shape[1] * 2 * opt->shape[2]);
+    d->fn_norm = norm;
+
+    double q = b->charge * kE;
+    double unit_conversion = kC / (1e9 * kE);
+    d->factor = opt->kick_scale * unit_conversion * q * time_step * d->fn_norm /
+                (b->ref_p * b->ref_gamma * b->ref_gamma * b->ref_beta);
+    return 0;
+}
+
+// collective/utils.h:9-24.  The guard keeps the int conversion defined for
+// positions far outside the grid (or NaN); every such particle is outside the
+// grid either way.
+extern "C" int32_t orc_cell_index(double pos, double left, double inv_h,
+                                  double* off)
+{
+    double s = (pos - left) * inv_h - 0.5;
+    if (!(s > -1.0e9 && s < 1.0e9)) {
+        *off = 0.0;
+        return -1000000000;
+    }
+    int32_t idx = static_cast<int32_t>(std::floor(s));
+    *off = s - idx;
+    return idx;
+}
+
+// Build-side cell key: k = (iz*gy + iy)*gx + ix for active particles whose
+// lower corner lies in [0, g-2]^3 (exactly the particles that the deposit
+// and the kick touch), else G = gx*gy*gz.
+extern "C" void orc_cell_keys(const orc_bunch* b, const orc_domain* d,
+                              const int32_t g[3], int32_t* keys)
+{
+    const double ih[3] = {1.0 / d->h[0], 1.0 / d->h[1], 1.0 / d->h[2]};
+    const int32_t G = g[0] * g[1] * g[2];
+    for (int64_t i = 0; i < b->n_active; ++i) {
+        double o;
+        int32_t ix = orc_cell_index(b->parts[i], d->left[0], ih[0], &o);
+        int32_t iy = orc_cell_index(b->parts[i + 2 * b->stride], d->left[1],
+                                    ih[1], &o);
+        int32_t iz = orc_cell_index(b->parts[i + 4 * b->stride], d->left[2],
+                                    ih[2], &o);
+        bool in = b->masks[i] && ix >= 0 && ix < g[0] - 1 && iy >= 0 &&
+                  iy < g[1] - 1 && iz >= 0 && iz < g[2] - 1;
+        keys[i] = in ? (iz * g[1] + iy) * g[0] + ix : G;
+    }
+}
+
+extern "C" void orc_stable_sort(const int32_t* keys, int64_t n, int32_t* perm)
+{
+    std::vector<int32_t> p(n);
+    std::iota(p.begin(), p.end(), 0);
+    std::stable_sort(p.begin(), p.end(),
+                     [keys](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+    std::memcpy(perm, p.data(), sizeof(int32_t) * n);
+}
+
+// ---------------------------------------------------------------------------
+// FFT.  The reference calls FFTW 3.3.10 (fftw_mpi_plan_dft_r2c_3d /
+// _c2r_3d on dims (Dz, Dy, Dx), distributed_fft3d_fftw.cc:67-71): forward
+// sign -1, unnormalised both ways, padded real rows of 2*(Dx/2+1).  Here:
+// iterative radix-2 Cooley-Tukey for powers of two, a direct O(n^2) DFT
+// otherwise; c2r uses the Hermitian extension of each x row with the
+// imaginary parts of the DC and Nyquist terms ignored, as FFTW does.
+// ---------------------------------------------------------------------------
+namespace {
+// Twiddle table exp(-2 pi i k / n), k < n/2, one per length and thread.
+const std::vector<double>& twiddles(int n)
+{
+    thread_local std::vector<std::vector<double>> cache(32);
+    int lg = 0;
+    while ((1 << lg) < n) ++lg;
+    std::vector<double>& t = cache[lg];
+    if (t.empty()) {
+        t.resize(n);  // n/2 complex entries
+        for (int k = 0; k < n / 2; ++k) {
+            double ang = -2.0 * kPi * double(k) / n;
+            t[2 * k] = std::cos(ang);
+            t[2 * k + 1] = std::sin(ang);
+        }
+    }
+    return t;
+}
+
+void fft_pow2(double* a, int n, int sign)
+{
+    const std::vector<double>& tw = twiddles(n);
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(a[2 * i], a[2 * j]);
+            std::swap(a[2 * i + 1], a[2 * j + 1]);
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len / 2;
+        int step = n / len;
+        for (int i = 0; i < n; i += len) {
+            for (int k = 0; k < half; ++k) {
+                // twiddle exp(sign * 2 pi i k / len) = table[k*step]
+                double wr = tw[2 * (k * step)];
+                double wi = sign < 0 ? tw[2 * (k * step) + 1]
+                                     : -tw[2 * (k * step) + 1];
+                double* u = a + 2 * (i + k);
+                double* v = a + 2 * (i + k + half);
+                double tr = v[0] * wr - v[1] * wi;
+                double ti = v[0] * wi + v[1] * wr;
+                v[0] = u[0] - tr;
+                v[1] = u[1] - ti;
+                u[0] = u[0] + tr;
+                u[1] = u[1] + ti;
+            }
+        }
+    }
+}
+
+void dft_direct(double* a, int n, int sign)
+{
+    std::vector<double> out(2 * n);
+    for (int k = 0; k < n; ++k) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < n; ++j) {
+            long long m = (static_cast<long long>(j) * k) % n;
+            double ang = sign * 2.0 * kPi * double(m) / n;
+            double wr = std::cos(ang), wi = std::sin(ang);
+            sr += a[2 * j] * wr - a[2 * j + 1] * wi;
+            si += a[2 * j] * wi + a[2 * j + 1] * wr;
+        }
+        out[2 * k] = sr;
+        out[2 * k + 1] = si;
+    }
+    std::memcpy(a, out.data(), sizeof(double) * 2 * n);
+}
+}  // namespace
+
+extern "C" void orc_fft1d(double* data, int n, int sign)
+{
+    if (n <= 1) return;
+    if ((n & (n - 1)) == 0)
+        fft_pow2(data, n, sign);
+    else
+        dft_direct(data, n, sign);
+}
+
+extern "C" void orc_fft3d_r2c(int nx, int ny, int nz, const double* in,
+                              double* out, int nthreads)
+{
+    const int px = padded_real(nx), hx = padded_cplx(nx);
+    const int64_t nrows = int64_t(ny) * nz;
+    // x rows: complex FFT of the real row, keep nx/2+1 outputs.
+    std::vector<double> tmp(size_t(nrows) * 2 * hx);
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<double> buf(2 * std::max(nx, std::max(ny, nz)));
+#pragma omp for schedule(static)
+        for (int64_t r = 0; r < nrows; ++r) {
+            const double* row = in + r * px;
+            for (int x = 0; x < nx; ++x) {
+                buf[2 * x] = row[x];
+                buf[2 * x + 1] = 0.0;
+            }
+            orc_fft1d(buf.data(), nx, -1);
+            std::memcpy(&tmp[size_t(r) * 2 * hx], buf.data(),
+                        sizeof(double) * 2 * hx);
+        }
+#pragma omp barrier
+        // y columns
+#pragma omp for schedule(static)
+        for (int64_t zc = 0; zc < int64_t(nz) * hx; ++zc) {
+            int64_t z = zc / hx, c = zc % hx;
+            for (int y = 0; y < ny; ++y) {
+                size_t idx = (size_t(z * ny + y) * hx + c) * 2;
+                buf[2 * y] = tmp[idx];
+                buf[2 * y + 1] = tmp[idx + 1];
+            }
+            orc_fft1d(buf.data(), ny, -1);
+            for (int y = 0; y < ny; ++y) {
+                size_t idx = (size_t(z * ny + y) * hx + c) * 2;
+                tmp[idx] = buf[2 * y];
+                tmp[idx + 1] = buf[2 * y + 1];
+            }
+        }
+#pragma omp barrier
+        // z columns
+#pragma omp for schedule(static)
+        for (int64_t yc = 0; yc < int64_t(ny) * hx; ++yc) {
+            for (int z = 0; z < nz; ++z) {
+                size_t idx = (size_t(z) * ny * hx + yc) * 2;
+                buf[2 * z] = tmp[idx];
+                buf[2 * z + 1] = tmp[idx + 1];
+            }
+            orc_fft1d(buf.data(), nz, -1);
+            for (int z = 0; z < nz; ++z) {
+                size_t idx = (size_t(z) * ny * hx + yc) * 2;
+                tmp[idx] = buf[2 * z];
+                tmp[idx + 1] = buf[2 * z + 1];
+            }
+        }
+    }
+    std::memcpy(out, tmp.data(), sizeof(double) * tmp.size());
+}
+
+extern "C" void orc_fft3d_c2r(int nx, int ny, int nz, const double* in,
+                              double* out, int nthreads)
+{
+    const int px = padded_real(nx), hx = padded_cplx(nx);
+    const int64_t nrows = int64_t(ny) * nz;
+    std::vector<double> tmp(in, in + size_t(nrows) * 2 * hx);
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<double> buf(2 * std::max(nx, std::max(ny, nz)));
+#pragma omp for schedule(static)
+        for (int64_t yc = 0; yc < int64_t(ny) * hx; ++yc) {
+            for (int z = 0; z < nz; ++z) {
+                size_t idx = (size_t(z) * ny * hx + yc) * 2;
+                buf[2 * z] = tmp[idx];
+                buf[2 * z + 1] = tmp[idx + 1];
+            }
+            orc_fft1d(buf.data(), nz, +1);
+            for (int z = 0; z < nz; ++z) {
+                size_t idx = (size_t(z) * ny * hx + yc) * 2;
+                tmp[idx] = buf[2 * z];
+                tmp[idx + 1] = buf[2 * z + 1];
+            }
+        }
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int64_t zc = 0; zc < int64_t(nz) * hx; ++zc) {
+            int64_t z = zc / hx, c = zc % hx;
+            for (int y = 0; y < ny; ++y) {
+                size_t idx = (size_t(z * ny + y) * hx + c) * 2;
+                buf[2 * y] = tmp[idx];
+                buf[2 * y + 1] = tmp[idx + 1];
+            }
+            orc_fft1d(buf.data(), ny, +1);
+            for (int y = 0; y < ny; ++y) {
+                size_t idx = (size_t(z * ny + y) * hx + c) * 2;
+                tmp[idx] = buf[2 * y];
+                tmp[idx + 1] = buf[2 * y + 1];
+            }
+        }
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int64_t r = 0; r < nrows; ++r) {
+            const double* X = &tmp[size_t(r) * 2 * hx];
+            for (int k = 0; k < hx; ++k) {
+                buf[2 * k] = X[2 * k];
+                buf[2 * k + 1] = X[2 * k + 1];
+            }
+            buf[1] = 0.0;
+            if (nx % 2 == 0) buf[2 * (nx / 2) + 1] = 0.0;
+            for (int k = hx; k < nx; ++k) {
+                buf[2 * k] = X[2 * (nx - k)];
+                buf[2 * k + 1] = -X[2 * (nx - k) + 1];
+            }
+            orc_fft1d(buf.data(), nx, +1);
+            double* row = out + r * px;
+            for (int x = 0; x < nx; ++x) row[x] = buf[2 * x];
+            for (int x = nx; x < px; ++x) row[x] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Green's functions on the padded doubled grid
+// (space_charge_3d_open_hockney.cc:136-348, launched :591-639), including the
+// reference's float-reciprocal index decomposition and mirror writes.
+// ---------------------------------------------------------------------------
+extern "C" void orc_green_fn2(int green_fn, const int32_t g[3],
+                              const double h[3], double* g2)
+{
+    const int dgx = 2 * g[0], dgy = 2 * g[1], dgz = 2 * g[2];
+    const int pdx = padded_real(dgx);
+    const int64_t tot = int64_t(pdx) * dgy * dgz;
+    std::fill(g2, g2 + tot, 0.0);
+    const double hx = h[0], hy = h[1], hz = h[2];
+    const int gx = g[0] + 1, gy = g[1] + 1;
+    const double igxgy = 1.0 / (gx * gy);
+    const double igx = 1.0 / gx;
+    auto at = [&](int iz, int iy, int ix) -> double& {
+        return g2[int64_t(iz) * pdx * dgy + int64_t(iy) * pdx + ix];
+    };
+    if (green_fn == 0) {
+        // alg_g2_pointlike (:136-228)
+        const double G000 = 2.8 / std::min(hx, std::min(hy, hz));
+        const int n = gx * gy * (g[2] + 1);
+        for (int i = 0; i < n; ++i) {
+            int iz = i * igxgy;
+            int iy = (i - iz * gx * gy) * igx;
+            int ix = i - iz * gx * gy - iy * gx;
+            double dx = ix * hx, dy = iy * hy, dz = iz * hz;
+            double G;
+            if (ix == 0 && iy == 0 && iz == 0)
+                G = G000;
+            else
+                G = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+            int mix = dgx - ix;
+            if (mix == dgx) mix = 0;
+            int miy = dgy - iy;
+            if (miy == dgy) miy = 0;
+            int miz = dgz - iz;
+            if (miz == dgz) miz = 0;
+            at(iz, iy, ix) = G;
+            at(iz, miy, ix) = G;
+            at(iz, iy, mix) = G;
+            at(iz, miy, mix) = G;
+            at(miz, iy, ix) = G;
+            at(miz, miy, ix) = G;
+            at(miz, iy, mix) = G;
+            at(miz, miy, mix) = G;
+        }
+        return;
+    }
+    // alg_g2_linear (:230-348)
+    const int gx0 = g[0], gy0 = g[1], gz0 = g[2];
+    double rr_h = hx * hx + hy * hy;
+    double r1_h = std::sqrt(hx * hx + hy * hy + hz * hz);
+    const double G000 = (2.0 / rr_h) *
+                        (hz * r1_h + rr_h * std::log((hz + r1_h) / std::sqrt(rr_h)) -
+                         hz * hz);
+    const int n = gx * gy * dgz;
+    for (int i = 0; i < n; ++i) {
+        int iz = i * igxgy;
+        int iy = (i - iz * gx * gy) * igx;
+        int ix = i - iz * gx * gy - iy * gx;
+        double z = (iz > gz0) ? (dgz - iz) * hz : iz * hz;
+        double y = iy * hy;
+        double x = ix * hx;
+        double rr = x * x + y * y;
+        double s_rr_0 = std::sqrt(rr + z * z);
+        double s_rr_n = std::sqrt(rr + (z - hz) * (z - hz));
+        double s_rr_p = std::sqrt(rr + (z + hz) * (z + hz));
+        double G = 2.0 * s_rr_0 - s_rr_n - s_rr_p;
+        double T1, T2, r1, r2;
+        const double epsz = 1e-12 * hz;
+        if (z < -hz) {
+            r1 = (s_rr_n - z + hz) / (s_rr_0 - z);
+            T1 = (hz - z) * std::log(r1);
+            r2 = (s_rr_0 - z) / (s_rr_p - z - hz);
+            T2 = (hz + z) * std::log(r2);
+            G += T1 + T2;
+        } else if (std::abs(z + hz) < epsz) {
+            r1 = (s_rr_n - z + hz) / (s_rr_0 - z);
+            T1 = (hz - z) * std::log(r1);
+            G += T1;
+        } else if (std::abs(z) < epsz) {
+            if (std::abs(x) + std::abs(y) < 2. * epsz) {
+                G += hz * G000;
+            } else {
+                r1 = (std::sqrt(hz * hz + rr) + hz) / std::sqrt(rr);
+                G += 2.0 * hz * std::log(r1);
+            }
+        } else if (std::abs(z - hz) < epsz) {
+            r1 = (s_rr_p + z + hz) / (s_rr_0 + z);
+            T1 = (hz + z) * std::log(r1);
+            G += T1;
+        } else if (z > hz) {
+            r1 = (s_rr_0 + z) / (s_rr_n + z - hz);
+            T1 = (hz - z) * std::log(r1);
+            r2 = (s_rr_p + z + hz) / (s_rr_0 + z);
+            T2 = (hz + z) * std::log(r2);
+            G += T1 + T2;
+        }
+        int miy = dgy - iy;
+        int mix = dgx - ix;
+        if (mix == dgx) mix = 0;
+        if (miy == dgy) miy = 0;
+        at(iz, iy, ix) = G;
+        if (iy != gy0) {
+            at(iz, miy, ix) = G;
+            if (ix != gx0) at(iz, miy, mix) = G;
+        }
+        if (ix != gx0) at(iz, iy, mix) = G;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CIC deposit, CPU form (deposit.cc:852-1031): per-thread private G grids,
+// static particle split, z-slab reduction into the padded doubled rho2.
+// ---------------------------------------------------------------------------
+namespace {
+inline bool ingrid(int ix, int iy, int iz, int gx, int gy, int gz)
+{
+    return ix > 0 && ix < gx - 1 && iy > 0 && iy < gy - 1 && iz > 0 &&
+           iz < gz - 1;  // deposit.cc:21-30
+}
+}  // namespace
+
+extern "C" void orc_deposit(const orc_bunch* b, const orc_domain* d,
+                            const int32_t g[3], double* rho2, int nthreads)
+{
+    const int gx = g[0], gy = g[1], gz = g[2];
+    const int dx = padded_real(2 * gx), dy = 2 * gy, dz = 2 * gz;
+    const int64_t nc = int64_t(gx) * gy * gz;
+    std::fill(rho2, rho2 + int64_t(dx) * dy * dz, 0.0);
+    const double w0 = d->w0;
+    const double lx = d->left[0], ly = d->left[1], lz = d->left[2];
+    const double ihx = 1.0 / d->h[0], ihy = 1.0 / d->h[1], ihz = 1.0 / d->h[2];
+    const int64_t npart = b->n_active;
+    const double* p = b->parts;
+    const int64_t st = b->stride;
+    std::vector<double> rl(size_t(nthreads) * nc);
+#pragma omp parallel num_threads(nthreads)
+    {
+        int nt = omp_get_num_threads();
+        int it = omp_get_thread_num();
+        int64_t le = npart / nt;
+        int64_t ps = it * le;
+        int64_t pe = (it == nt - 1) ? npart : (it + 1) * le;
+        double* r = rl.data() + size_t(it) * nc;
+        std::memset(r, 0, sizeof(double) * nc);
+        for (int64_t n = ps; n < pe; ++n) {
+            if (!b->masks[n]) continue;
+            double ox, oy, oz;
+            int ix = orc_cell_index(p[n], lx, ihx, &ox);
+            int iy = orc_cell_index(p[n + 2 * st], ly, ihy, &oy);
+            int iz = orc_cell_index(p[n + 4 * st], lz, ihz, &oz);
+            int64_t base = int64_t(iz) * gx * gy;
+            if (ingrid(ix, iy, iz, gx, gy, gz))
+                r[base + iy * gx + ix] += w0 * (1 - ox) * (1 - oy) * (1 - oz);
+            if (ingrid(ix + 1, iy, iz, gx, gy, gz))
+                r[base + iy * gx + ix + 1] += w0 * (ox) * (1 - oy) * (1 - oz);
+            if (ingrid(ix, iy + 1, iz, gx, gy, gz))
+                r[base + (iy + 1) * gx + ix] += w0 * (1 - ox) * (oy) * (1 - oz);
+            if (ingrid(ix + 1, iy + 1, iz, gx, gy, gz))
+                r[base + (iy + 1) * gx + ix + 1] += w0 * (ox) * (oy) * (1 - oz);
+            base = int64_t(iz + 1) * gx * gy;
+            if (ingrid(ix, iy, iz + 1, gx, gy, gz))
+                r[base + iy * gx + ix] += w0 * (1 - ox) * (1 - oy) * (oz);
+            if (ingrid(ix + 1, iy, iz + 1, gx, gy, gz))
+                r[base + iy * gx + ix + 1] += w0 * (ox) * (1 - oy) * (oz);
+            if (ingrid(ix, iy + 1, iz + 1, gx, gy, gz))
+                r[base + (iy + 1) * gx + ix] += w0 * (1 - ox) * (oy) * (oz);
+            if (ingrid(ix + 1, iy + 1, iz + 1, gx, gy, gz))
+                r[base + (iy + 1) * gx + ix + 1] += w0 * (ox) * (oy) * (oz);
+        }
+#pragma omp barrier
+        int zle = gz / nt;
+        int zps = it * zle;
+        int zpe = (it == nt - 1) ? gz : (zps + zle);
+        for (int z = zps; z < zpe; ++z)
+            for (int y = 0; y < gy; ++y)
+                for (int x = 0; x < gx; ++x) {
+                    double w = 0.0;
+                    for (int t = 0; t < nt; ++t)
+                        w += rl[size_t(t) * nc + (int64_t(z) * gx * gy + y * gx + x)];
+                    rho2[int64_t(z) * dx * dy + int64_t(y) * dx + x] = w;
+                }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E field, zyx::alg_force_extractor_doubled_domain
+// (space_charge_3d_kernels.h:217-307): interior nodes only.
+// ---------------------------------------------------------------------------
+extern "C" void orc_force(const double* phi2, const int32_t g[3],
+                          const double h[3], double* enx, double* eny,
+                          double* enz)
+{
+    const int gx = g[0], gy = g[1], gz = g[2];
+    const int dgx = padded_real(2 * gx), dgy = 2 * gy;
+    const double ihx = 0.5 / h[0], ihy = 0.5 / h[1], ihz = 0.5 / h[2];
+    const double igxgy = 1.0 / (gx * gy);
+    const double igx = 1.0 / gx;
+    const int n = gx * gy * gz;
+    std::fill(enx, enx + n, 0.0);
+    std::fill(eny, eny + n, 0.0);
+    std::fill(enz, enz + n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        int iz = i * igxgy;
+        int iy = (i - iz * gx * gy) * igx;
+        int ix = i - iz * gx * gy - iy * gx;
+        if (ix == 0 || ix == gx - 1 || iy == 0 || iy == gy - 1 || iz == 0 ||
+            iz == gz - 1)
+            continue;
+        int64_t r = int64_t(iz) * dgx * dgy + int64_t(iy) * dgx + (ix + 1);
+        int64_t l = int64_t(iz) * dgx * dgy + int64_t(iy) * dgx + (ix - 1);
+        enx[i] = -(phi2[r] - phi2[l]) * ihx;
+        r = int64_t(iz) * dgx * dgy + int64_t(iy + 1) * dgx + ix;
+        l = int64_t(iz) * dgx * dgy + int64_t(iy - 1) * dgx + ix;
+        eny[i] = -(phi2[r] - phi2[l]) * ihy;
+        r = int64_t(iz + 1) * dgx * dgy + int64_t(iy) * dgx + ix;
+        l = int64_t(iz - 1) * dgx * dgy + int64_t(iy) * dgx + ix;
+        enz[i] = -(phi2[r] - phi2[l]) * ihz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kick, zyx::alg_kicker (space_charge_3d_kernels.h:91-215).
+// ---------------------------------------------------------------------------
+extern "C" void orc_kick(orc_bunch* b, const double* enx, const double* eny,
+                         const double* enz, const int32_t g[3],
+                         const orc_domain* d)
+{
+    const int gx = g[0], gy = g[1], gz = g[2];
+    const double ihx = 1.0 / d->h[0], ihy = 1.0 / d->h[1], ihz = 1.0 / d->h[2];
+    const double lx = d->left[0], ly = d->left[1], lz = d->left[2];
+    const double factor = d->factor, pref = b->ref_p, m = b->mass;
+    double* p = b->parts;
+    const int64_t st = b->stride;
+    const int64_t n = b->n_active;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        if (!b->masks[i]) continue;
+        double ox, oy, oz;
+        int ix = orc_cell_index(p[i], lx, ihx, &ox);
+        int iy = orc_cell_index(p[i + 2 * st], ly, ihy, &oy);
+        int iz = orc_cell_index(p[i + 4 * st], lz, ihz, &oz);
+        double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
+        if (!((ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) &&
+              (iz >= 0 && iz < gz - 1)))
+            continue;
+        auto interp = [&](const double* e) {
+            int64_t base = int64_t(iz) * gx * gy + int64_t(iy) * gx + ix;
+            double val = aoz * aoy * aox * e[base];
+            val += aoz * aoy * ox * e[base + 1];
+            val += aoz * oy * aox * e[base + gx];
+            val += aoz * oy * ox * e[base + gx + 1];
+            base = int64_t(iz + 1) * gx * gy + int64_t(iy) * gx + ix;
+            val += oz * aoy * aox * e[base];
+            val += oz * aoy * ox * e[base + 1];
+            val += oz * oy * aox * e[base + gx];
+            val += oz * oy * ox * e[base + gx + 1];
+            return val;
+        };
+        double val = interp(enz);
+        double pp = pref + p[i + 5 * st] * pref;
+        double Eoc_i = std::sqrt(pp * pp + m * m);
+        double Eoc_f = Eoc_i + factor * (-pref) * val;
+        double dpop =
+            (std::sqrt(Eoc_f * Eoc_f - m * m) - std::sqrt(Eoc_i * Eoc_i - m * m)) /
+            pref;
+        p[i + 5 * st] += dpop;
+        val = interp(eny);
+        p[i + 3 * st] += factor * val;
+        val = interp(enx);
+        p[i + 1 * st] += factor * val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One operator application on one bunch (apply_bunch, .cc:422-453).
+// ---------------------------------------------------------------------------
+extern "C" int orc_apply(const orc_options* opt, orc_bunch* b, double time_step,
+                         int nthreads, orc_debug* dbg)
+{
+    try {
+        if (nthreads <= 0) nthreads = omp_get_max_threads();
+        omp_set_num_threads(nthreads);
+        const int32_t g[3] = {opt->shape[0], opt->shape[1], opt->shape[2]};
+        const int dgx = 2 * g[0], dgy = 2 * g[1], dgz = 2 * g[2];
+        const int pdx = padded_real(dgx);
+        const int64_t tot = int64_t(pdx) * dgy * dgz;
+        const int64_t G = int64_t(g[0]) * g[1] * g[2];
+        double t0 = wtime(), t1;
+        double secs[6] = {0, 0, 0, 0, 0, 0};
+
+        orc_domain d;
+        int rc = orc_compute_domain(opt, b, time_step, &d);
+        if (rc) return rc;
+        t1 = wtime(); secs[0] = t1 - t0; t0 = t1;
+
+        if (opt->reorder || (dbg && (dbg->keys || dbg->perm))) {
+            const int64_t n = b->n_active;
+            std::vector<int32_t> keys(n), perm(n);
+            orc_cell_keys(b, &d, g, keys.data());
+            orc_stable_sort(keys.data(), n, perm.data());
+            if (dbg && dbg->keys) std::memcpy(dbg->keys, keys.data(), sizeof(int32_t) * n);
+            if (dbg && dbg->perm) std::memcpy(dbg->perm, perm.data(), sizeof(int32_t) * n);
+            if (opt->reorder) {
+                std::vector<double> col(n);
+                for (int c = 0; c < 7; ++c) {
+                    double* pc = b->parts + c * b->stride;
+                    for (int64_t j = 0; j < n; ++j) col[j] = pc[perm[j]];
+                    std::memcpy(pc, col.data(), sizeof(double) * n);
+                }
+                std::vector<uint8_t> m(n);
+                for (int64_t j = 0; j < n; ++j) m[j] = b->masks[perm[j]];
+                std::memcpy(b->masks, m.data(), n);
+            }
+            t0 = wtime();
+        }
+
+        std::vector<double> rho2(tot), g2(tot), phi2(tot);
+        orc_deposit(b, &d, g, rho2.data(), nthreads);
+        if (dbg && dbg->rho) {
+            for (int z = 0; z < g[2]; ++z)
+                for (int y = 0; y < g[1]; ++y)
+                    for (int x = 0; x < g[0]; ++x)
+                        dbg->rho[(int64_t(z) * g[1] + y) * g[0] + x] =
+                            rho2[int64_t(z) * pdx * dgy + int64_t(y) * pdx + x];
+        }
+        if (dbg && dbg->rho2_full)
+            std::memcpy(dbg->rho2_full, rho2.data(), sizeof(double) * tot);
+        t1 = wtime(); secs[1] = t1 - t0; t0 = t1;
+
+        if (opt->periodic_z) {
+            g_err = opt->green_fn == 0
+                        ? "Space_charge_3d_open_hockney::get_green_fn2_pointlike: "
+                          "periodic_z not yet implemented"
+                        : "Space_charge_3d_open_hockney::get_green_fn2_linear: "
+                          "periodic_z not yet implemented";
+            return -4;
+        }
+        orc_green_fn2(opt->green_fn, g, d.h, g2.data());
+        if (dbg && dbg->g2_full)
+            std::memcpy(dbg->g2_full, g2.data(), sizeof(double) * tot);
+        t1 = wtime(); secs[2] = t1 - t0; t0 = t1;
+
+        orc_fft3d_r2c(dgx, dgy, dgz, rho2.data(), rho2.data(), nthreads);
+        orc_fft3d_r2c(dgx, dgy, dgz, g2.data(), g2.data(), nthreads);
+        if (dbg && dbg->ghat_full)
+            std::memcpy(dbg->ghat_full, g2.data(), sizeof(double) * tot);
+        const int64_t ncplx = int64_t(dgz) * dgy * padded_cplx(dgx);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+        for (int64_t i = 0; i < ncplx; ++i) {
+            // alg_cplx_multiplier (.cc:350-372)
+            const double ar = rho2[2 * i], ai = rho2[2 * i + 1];
+            const double br = g2[2 * i], bi = g2[2 * i + 1];
+            phi2[2 * i] = ar * br - ai * bi;
+            phi2[2 * i + 1] = ar * bi + ai * br;
+        }
+        orc_fft3d_c2r(dgx, dgy, dgz, phi2.data(), phi2.data(), nthreads);
+        t1 = wtime(); secs[3] = t1 - t0; t0 = t1;
+
+        std::vector<double> ex(G), ey(G), ez(G);
+        orc_force(phi2.data(), g, d.h, ex.data(), ey.data(), ez.data());
+        t1 = wtime(); secs[4] = t1 - t0; t0 = t1;
+
+        orc_kick(b, ex.data(), ey.data(), ez.data(), g, &d);
+        t1 = wtime(); secs[5] = t1 - t0;
+
+        if (dbg) {
+            if (dbg->domain) *dbg->domain = d;
+            for (int k = 0; k < 6; ++k) dbg->seconds[k] = secs[k];
+            auto octant = [&](const std::vector<double>& src, double* dst) {
+                for (int z = 0; z < g[2]; ++z)
+                    for (int y = 0; y < g[1]; ++y)
+                        for (int x = 0; x < g[0]; ++x)
+                            dst[(int64_t(z) * g[1] + y) * g[0] + x] =
+                                src[int64_t(z) * pdx * dgy + int64_t(y) * pdx + x];
+            };
+            if (dbg->phi) octant(phi2, dbg->phi);
+            if (dbg->ex) std::memcpy(dbg->ex, ex.data(), sizeof(double) * G);
+            if (dbg->ey) std::memcpy(dbg->ey, ey.data(), sizeof(double) * G);
+            if (dbg->ez) std::memcpy(dbg->ez, ez.data(), sizeof(double) * G);
+        }
+        return 0;
+    } catch (std::exception const& e) {
+        g_err = e.what();
+        return -1;
+    }
+}

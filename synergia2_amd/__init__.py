@@ -1,0 +1,289 @@
+"""synergia2_amd -- MI355X-native 3D open-boundary Hockney space-charge step.
+
+Python binding of the C ABI in include/sc3d_capi.h (libsc3d_hip.so) over
+ctypes.  Device memory comes from torch (ROCm) tensors; the library only sees
+raw device pointers.  There is no CPU fallback: if the HIP library is not
+built, load() raises.
+
+Mirrors the reference operator interface:
+  Space_charge_3d_open_hockney_options  (implemented_collective_options.h:67-126)
+  Space_charge_3d_open_hockney          (space_charge_3d_open_hockney.h:16-73)
+  Bunch columns x, xp, y, yp, cdt, dpop, id (bunch.h:66-74)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsc3d_hip.so")
+
+# physical_constants.h:17-24 (PDG 2012), :94-106
+PROTON_MASS = 0.938272046
+E_CHARGE = 1.602176565e-19
+C_LIGHT = 299792458.0
+
+# bunch.h:66-74
+X, XP, Y, YP, CDT, DPOP, ID = range(7)
+
+SC3D_OK = 0
+SC3D_ERR_ARG = -1
+SC3D_ERR_HIP = -2
+SC3D_ERR_DOMAIN = -3
+SC3D_ERR_PERIODIC = -4
+SC3D_ERR_SIZE = -5
+
+GREEN_POINTLIKE = 0
+GREEN_LINEAR = 1
+FP64 = 0
+FP32_GRID = 1
+
+DBG_DOMAIN, DBG_KEYS, DBG_PERM, DBG_RHO, DBG_PHI, DBG_EX, DBG_EY, DBG_EZ, DBG_GHAT = range(9)
+
+
+class Sc3dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32 * 3),
+        ("size", C.c_double * 3),
+        ("offset", C.c_double * 3),
+        ("green_fn", C.c_int32),
+        ("periodic_z", C.c_int32),
+        ("grid_entire_period", C.c_int32),
+        ("domain_fixed", C.c_int32),
+        ("comm_group_size", C.c_int32),
+        ("z_period", C.c_double),
+        ("n_sigma", C.c_double),
+        ("kick_scale", C.c_double),
+        ("precision", C.c_int32),
+        ("sort_period", C.c_int32),
+    ]
+
+
+class BunchDesc(C.Structure):
+    _fields_ = [
+        ("parts", C.c_void_p),
+        ("stride", C.c_int64),
+        ("n_active", C.c_int64),
+        ("masks", C.c_void_p),
+        ("total_num", C.c_double),
+        ("real_num", C.c_double),
+        ("charge", C.c_int32),
+        ("mass", C.c_double),
+        ("ref_gamma", C.c_double),
+        ("ref_beta", C.c_double),
+        ("ref_p", C.c_double),
+    ]
+
+
+class DomainRecord(C.Structure):
+    _fields_ = [
+        ("sums", C.c_double * 6),
+        ("mean", C.c_double * 3),
+        ("stddev", C.c_double * 3),
+        ("size", C.c_double * 3),
+        ("offset", C.c_double * 3),
+        ("left", C.c_double * 3),
+        ("h", C.c_double * 3),
+        ("w0", C.c_double),
+        ("fn_norm", C.c_double),
+        ("factor", C.c_double),
+        ("inv_h", C.c_double * 3),
+        ("half_inv_h", C.c_double * 3),
+        ("ghat_h", C.c_double * 3),
+        ("error", C.c_int32),
+        ("ghat_dirty", C.c_int32),
+        ("ghat_valid", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
+           "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian")
+
+_lib = None
+
+
+def load(path=None):
+    """Load libsc3d_hip.so (raises if it is not built; there is no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is not built; run __graft_entry__.build() or "
+                           "python -m synergia2_amd.build")
+    L = C.CDLL(p)
+    L.sc3d_options_default.argtypes = [C.POINTER(Options), C.c_int32, C.c_int32, C.c_int32]
+    L.sc3d_options_default.restype = None
+    L.sc3d_create.argtypes = [C.POINTER(Options), C.POINTER(C.c_void_p)]
+    L.sc3d_apply.argtypes = [C.c_void_p, C.POINTER(BunchDesc), C.c_double, C.c_void_p]
+    L.sc3d_sync.argtypes = [C.c_void_p]
+    L.sc3d_destroy.argtypes = [C.c_void_p]
+    L.sc3d_last_error.restype = C.c_char_p
+    L.sc3d_debug_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
+    L.sc3d_populate_gaussian.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.c_uint64, C.c_int64, C.c_void_p]
+    for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
+                 "sc3d_populate_gaussian"):
+        getattr(L, name).restype = C.c_int
+    if path is None:
+        _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != SC3D_OK:
+        raise Sc3dError(rc, load().sc3d_last_error().decode())
+
+
+def default_options(gx=32, gy=32, gz=64):
+    """Space_charge_3d_open_hockney_options(gridx, gridy, gridz) defaults."""
+    o = Options()
+    load().sc3d_options_default(C.byref(o), gx, gy, gz)
+    return o
+
+
+def set_fixed_domain(opts, offset, size):
+    """Space_charge_3d_open_hockney_options::set_fixed_domain
+    (implemented_collective_options.h:97-109); its grid_entire_period branch
+    is overwritten by the assignments that follow it, so it has no effect."""
+    opts.offset[:] = list(offset)
+    opts.size[:] = list(size)
+    opts.domain_fixed = 1
+
+
+def kinematics(mass, gamma):
+    """Four_momentum::update_from_gamma (four_momentum.cc:9-17)."""
+    beta = np.sqrt(1.0 - 1.0 / (gamma * gamma))
+    return float(beta), float(gamma * beta * mass)
+
+
+class Bunch:
+    """Device bunch: a (7, capacity) float64 torch tensor (column j contiguous,
+    the reference's LayoutLeft particle view) and a uint8 mask per slot."""
+
+    def __init__(self, n, real_num, mass=PROTON_MASS, charge=1, gamma=None,
+                 kinetic_energy=None, capacity=None, device="cuda"):
+        import torch
+        if gamma is None:
+            gamma = (mass + kinetic_energy) / mass
+        self.mass = float(mass)
+        self.charge = int(charge)
+        self.gamma = float(gamma)
+        self.beta, self.p = kinematics(self.mass, self.gamma)
+        self.n = int(n)
+        self.capacity = int(capacity or n)
+        self.total_num = float(n)
+        self.real_num = float(real_num)
+        self.parts = torch.zeros((7, max(self.capacity, 1)), dtype=torch.float64, device=device)
+        self.masks = torch.ones(max(self.capacity, 1), dtype=torch.uint8, device=device)
+
+    def desc(self):
+        d = BunchDesc()
+        d.parts = self.parts.data_ptr()
+        d.stride = self.parts.shape[1]
+        d.n_active = self.n
+        d.masks = self.masks.data_ptr()
+        d.total_num = self.total_num
+        d.real_num = self.real_num
+        d.charge = self.charge
+        d.mass = self.mass
+        d.ref_gamma = self.gamma
+        d.ref_beta = self.beta
+        d.ref_p = self.p
+        return d
+
+    def set_host(self, parts_np, masks_np=None):
+        import torch
+        n = parts_np.shape[1]
+        self.parts[:, :n].copy_(torch.from_numpy(np.ascontiguousarray(parts_np)))
+        if masks_np is not None:
+            self.masks[:n].copy_(torch.from_numpy(np.ascontiguousarray(masks_np)))
+
+    def host(self):
+        return self.parts[:, :self.n].cpu().numpy().copy(), self.masks[:self.n].cpu().numpy().copy()
+
+    def populate_gaussian(self, mean, chol, seed, id0=0, stream=None):
+        mean = np.ascontiguousarray(mean, np.float64)
+        chol = np.ascontiguousarray(chol, np.float64)
+        assert mean.shape == (6,) and chol.shape == (21,)
+        _check(load().sc3d_populate_gaussian(
+            self.parts.data_ptr(), self.parts.shape[1], self.n, self.masks.data_ptr(),
+            mean.ctypes.data_as(C.POINTER(C.c_double)), chol.ctypes.data_as(C.POINTER(C.c_double)),
+            int(seed), int(id0), _stream_ptr(stream)))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def diag_chol(sigmas):
+    """Packed lower-triangular factor of an uncorrelated 6D Gaussian."""
+    L = np.zeros(21)
+    for r in range(6):
+        L[r * (r + 1) // 2 + r] = sigmas[r]
+    return L
+
+
+class SpaceCharge3dOpenHockney:
+    """Space_charge_3d_open_hockney over the C ABI: one context (device
+    workspaces + cached Green's spectrum) per operator instance."""
+
+    def __init__(self, options):
+        self.options = options
+        self._ctx = C.c_void_p()
+        _check(load().sc3d_create(C.byref(options), C.byref(self._ctx)))
+        g = list(options.shape)
+        self.G = g[0] * g[1] * g[2]
+        self.ghat_elems = (g[2] + 1) * (g[1] + 1) * (g[0] + 1)
+
+    def apply_bunch(self, bunch, time_step, stream=None, sync=True):
+        d = bunch.desc()
+        _check(load().sc3d_apply(self._ctx, C.byref(d), float(time_step), _stream_ptr(stream)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        _check(load().sc3d_sync(self._ctx))
+
+    def debug(self, what, n=None):
+        if what == DBG_DOMAIN:
+            rec = DomainRecord()
+            _check(load().sc3d_debug_get(self._ctx, what, C.byref(rec), C.sizeof(rec)))
+            return rec.as_dict()
+        if what in (DBG_KEYS, DBG_PERM):
+            out = np.zeros(n, np.int32)
+        elif what == DBG_GHAT:
+            out = np.zeros(self.ghat_elems)
+        else:
+            out = np.zeros(self.G)
+        _check(load().sc3d_debug_get(self._ctx, what, out.ctypes.data, out.nbytes))
+        return out
+
+    def close(self):
+        if self._ctx:
+            load().sc3d_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

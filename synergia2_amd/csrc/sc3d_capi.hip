@@ -1,0 +1,406 @@
+// sc3d_capi.hip -- the C ABI of libsc3d_hip.so (declared in
+// include/sc3d_capi.h): context = device workspaces + cached Green's
+// spectrum of one operator instance, and the per-apply launch sequence of
+// Space_charge_3d_open_hockney::apply_bunch
+// (space_charge_3d_open_hockney.cc:422-453):
+//   domain -> [cell sort + reorder] -> deposit -> Green's spectrum (only when
+//   h changed) -> x/y/z FFT passes with the convolution fused into the z pass
+//   -> E field -> kick.
+// Everything is enqueued on the caller's stream; no host synchronisation
+// happens inside sc3d_apply (the domain is finished on the device).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sc3d_particles.h"
+
+using namespace sc3d;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(SC3D_ERR_HIP, std::string(#expr " failed: ") +                 \
+                                          hipGetErrorString(e_));                      \
+    } while (0)
+}  // namespace
+
+struct sc3d_ctx {
+    sc3d_options opt;
+    Geometry g;
+    int device = 0;
+    bool fp32 = false;
+    DomainRec* dom = nullptr;
+    double* P = nullptr;
+    int64_t P_cap = 0;
+    void* rho = nullptr;
+    void* S = nullptr;
+    void* phi = nullptr;
+    void* E = nullptr;
+    double* ghat64 = nullptr;
+    float* ghat32 = nullptr;
+    double2* tw64 = nullptr;
+    float2* tw32 = nullptr;
+    int64_t pcap = 0;
+    SortBuffers sb{};
+    double* scratch = nullptr;
+    uint8_t* smask = nullptr;
+    int64_t applies = 0;
+    int64_t sorted_n = -1;
+    hipStream_t last_stream = nullptr;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+template <typename T>
+int dmalloc(sc3d_ctx* c, T** p, int64_t n)
+{
+    void* q = nullptr;
+    if (n <= 0) n = 1;
+    HIP_TRY(hipMalloc(&q, size_t(n) * sizeof(T)));
+    c->owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+void dfree(sc3d_ctx* c, void* p)
+{
+    if (!p) return;
+    for (auto& q : c->owned)
+        if (q == p) {
+            (void)hipFree(q);
+            q = nullptr;
+        }
+}
+
+int ensure_particle_buffers(sc3d_ctx* c, int64_t n)
+{
+    if (n <= c->pcap) return 0;
+    int64_t cap = n + n / 8 + 4096;
+    for (void* p : {(void*)c->sb.keys0, (void*)c->sb.keysA, (void*)c->sb.keysB,
+                    (void*)c->sb.valsA, (void*)c->sb.valsB, (void*)c->sb.counts,
+                    (void*)c->sb.totals, (void*)c->scratch, (void*)c->smask})
+        dfree(c, p);
+    int64_t tiles = (cap + kSortTileItems - 1) / kSortTileItems;
+    int rc = 0;
+    rc |= dmalloc(c, &c->sb.keys0, cap);
+    rc |= dmalloc(c, &c->sb.keysA, cap);
+    rc |= dmalloc(c, &c->sb.keysB, cap);
+    rc |= dmalloc(c, &c->sb.valsA, cap);
+    rc |= dmalloc(c, &c->sb.valsB, cap);
+    rc |= dmalloc(c, &c->sb.counts, 256 * tiles);
+    rc |= dmalloc(c, &c->sb.totals, 256);
+    rc |= dmalloc(c, &c->scratch, 7 * cap);
+    rc |= dmalloc(c, &c->smask, cap);
+    if (rc) return rc;
+    c->pcap = cap;
+    return 0;
+}
+
+void make_twiddles(std::vector<double2>& t64, std::vector<float2>& t32)
+{
+    t64.resize(kTwiddleElems);
+    t32.resize(kTwiddleElems);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int L = 1; L <= kMaxLen; L *= 2)
+        for (int k = 0; k < L; ++k) {
+            long double a = -2.0L * pi * k / L;
+            long double cr = std::cos(a), si = std::sin(a);
+            // exact values on the axes
+            if (4 * k == L) { cr = 0; si = -1; }
+            if (2 * k == L) { cr = -1; si = 0; }
+            if (4 * k == 3 * L) { cr = 0; si = 1; }
+            if (k == 0) { cr = 1; si = 0; }
+            t64[L - 1 + k] = make_double2(double(cr), double(si));
+            t32[L - 1 + k] = make_float2(float(cr), float(si));
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sc3d_last_error(void) { return g_err.c_str(); }
+
+void sc3d_options_default(sc3d_options* o, int32_t gx, int32_t gy, int32_t gz)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->shape[0] = gx;
+    o->shape[1] = gy;
+    o->shape[2] = gz;
+    o->green_fn = SC3D_GREEN_LINEAR;
+    o->periodic_z = 0;
+    o->z_period = 0.0;
+    o->grid_entire_period = 0;
+    o->n_sigma = 8.0;
+    o->kick_scale = 1.0;
+    o->domain_fixed = 0;
+    o->comm_group_size = 4;
+    o->precision = SC3D_PRECISION_FP64;
+    o->sort_period = 1;
+}
+
+int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
+{
+    if (!opt || !out) return fail(SC3D_ERR_ARG, "sc3d_create: null argument");
+    *out = nullptr;
+    const int gx = opt->shape[0], gy = opt->shape[1], gz = opt->shape[2];
+    if (!pow2_in(gx, 8, 256) || !pow2_in(gy, 4, 256) || !pow2_in(gz, 4, 256))
+        return fail(SC3D_ERR_ARG,
+                    "sc3d_create: grid shape must be powers of two with gx in [8,256] and "
+                    "gy, gz in [4,256]");
+    if (opt->green_fn != SC3D_GREEN_LINEAR && opt->green_fn != SC3D_GREEN_POINTLIKE)
+        return fail(SC3D_ERR_ARG, "sc3d_create: unknown green_fn");
+    if (opt->precision != SC3D_PRECISION_FP64 && opt->precision != SC3D_PRECISION_FP32_GRID)
+        return fail(SC3D_ERR_ARG, "sc3d_create: unknown precision");
+    if (opt->sort_period < 0) return fail(SC3D_ERR_ARG, "sc3d_create: sort_period < 0");
+    sc3d_ctx* c = new sc3d_ctx();
+    c->opt = *opt;
+    c->fp32 = opt->precision == SC3D_PRECISION_FP32_GRID;
+    Geometry& g = c->g;
+    g.gx = gx; g.gy = gy; g.gz = gz;
+    g.Dx = 2 * gx; g.Dy = 2 * gy; g.Dz = 2 * gz;
+    g.Hx = gx + 1;
+    g.G = int64_t(gx) * gy * gz;
+    g.S_elems = int64_t(gz) * g.Dy * g.Hx;
+    g.Ghat_elems = int64_t(gz + 1) * (gy + 1) * g.Hx;
+    auto cleanup = [&](int rc) {
+        sc3d_destroy(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess)
+        return cleanup(fail(SC3D_ERR_HIP, "sc3d_create: no HIP device"));
+    const int64_t rs = c->fp32 ? 4 : 8;
+    int rc = 0;
+    rc |= dmalloc(c, &c->dom, 1);
+    char* p = nullptr;
+    rc |= dmalloc(c, &p, g.G * rs);
+    c->rho = p;
+    rc |= dmalloc(c, &p, g.S_elems * 2 * rs);
+    c->S = p;
+    rc |= dmalloc(c, &p, g.G * rs);
+    c->phi = p;
+    rc |= dmalloc(c, &p, g.G * 4 * rs);
+    c->E = p;
+    rc |= dmalloc(c, &c->ghat64, g.Ghat_elems);
+    if (c->fp32) rc |= dmalloc(c, &c->ghat32, g.Ghat_elems);
+    rc |= dmalloc(c, &c->tw64, kTwiddleElems);
+    rc |= dmalloc(c, &c->tw32, kTwiddleElems);
+    if (rc) return cleanup(rc);
+    std::vector<double2> t64;
+    std::vector<float2> t32;
+    make_twiddles(t64, t32);
+    if (hipMemcpy(c->tw64, t64.data(), t64.size() * sizeof(double2), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->tw32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemset(c->dom, 0, sizeof(DomainRec)) != hipSuccess)
+        return cleanup(fail(SC3D_ERR_HIP, "sc3d_create: workspace initialisation failed"));
+    *out = c;
+    return SC3D_OK;
+}
+
+int sc3d_destroy(sc3d_ctx* c)
+{
+    if (!c) return SC3D_OK;
+    if (c->last_stream || c->applies) (void)hipStreamSynchronize(c->last_stream);
+    for (void* q : c->owned)
+        if (q) (void)hipFree(q);
+    delete c;
+    return SC3D_OK;
+}
+
+int sc3d_apply(sc3d_ctx* c, const sc3d_bunch* b, double time_step, void* stream)
+{
+    if (!c || !b) return fail(SC3D_ERR_ARG, "sc3d_apply: null argument");
+    const int64_t n = b->n_active;
+    if (n < 0 || b->stride < n) return fail(SC3D_ERR_ARG, "sc3d_apply: bad n_active/stride");
+    if (n > 0 && (!b->parts || !b->masks))
+        return fail(SC3D_ERR_ARG, "sc3d_apply: null particle or mask pointer");
+    if (n >= (int64_t(1) << 31) - kSortTileItems)
+        return fail(SC3D_ERR_SIZE, "sc3d_apply: more than 2^31 particles in one bunch");
+    if (c->opt.periodic_z)
+        return fail(SC3D_ERR_PERIODIC,
+                    c->opt.green_fn == SC3D_GREEN_POINTLIKE
+                        ? "Space_charge_3d_open_hockney::get_green_fn2_pointlike: periodic_z "
+                          "not yet implemented"
+                        : "Space_charge_3d_open_hockney::get_green_fn2_linear: periodic_z not "
+                          "yet implemented");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const Geometry& g = c->g;
+    const sc3d_options& o = c->opt;
+
+    // bunch extent -> device domain record
+    const int64_t nch = (n + 4095) / 4096;
+    if (!o.domain_fixed && nch * 6 > c->P_cap) {
+        dfree(c, c->P);
+        if (int rc = dmalloc(c, &c->P, nch * 6 + 64)) return rc;
+        c->P_cap = nch * 6 + 64;
+    }
+    DomainFinishArgs a{};
+    a.fixed = o.domain_fixed;
+    a.green_fn = o.green_fn;
+    a.grid_entire_period = o.grid_entire_period;
+    a.charge = b->charge;
+    for (int k = 0; k < 3; ++k) {
+        a.shape[k] = o.shape[k];
+        a.size[k] = o.size[k];
+        a.offset[k] = o.offset[k];
+    }
+    a.n_sigma = o.n_sigma;
+    a.z_period = o.z_period;
+    a.kick_scale = o.kick_scale;
+    a.total_num = b->total_num;
+    a.real_num = b->real_num;
+    a.time_step = time_step;
+    a.ref_p = b->ref_p;
+    a.ref_gamma = b->ref_gamma;
+    a.ref_beta = b->ref_beta;
+    launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, st);
+
+    // cell sort + in-place reorder of the bunch
+    if (o.sort_period > 0 && (c->applies % o.sort_period) == 0 && n > 0) {
+        if (int rc = ensure_particle_buffers(c, n)) return rc;
+        launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb, st);
+        launch_reorder(b->parts, b->stride, b->masks, n, c->sb.perm, c->scratch, c->smask,
+                       c->dom, st);
+        c->sorted_n = n;
+    }
+
+    if (!c->fp32) {
+        double* rho = static_cast<double*>(c->rho);
+        launch_fill<double>(rho, g.G, st);
+        launch_deposit<double>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, rho,
+                               st);
+        launch_green<double>(g, o.green_fn, c->dom, c->tw64, c->ghat64, c->ghat64, st);
+        launch_solve<double>(g, rho, static_cast<double2*>(c->S), c->ghat64, c->tw64,
+                             static_cast<double*>(c->phi), c->dom, st);
+        launch_efield<double>(static_cast<double*>(c->phi), g.gx, g.gy, g.gz, c->dom,
+                              static_cast<double4*>(c->E), st);
+        launch_kick<double>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom,
+                            static_cast<double4*>(c->E), b->ref_p, b->mass, st);
+    } else {
+        float* rho = static_cast<float*>(c->rho);
+        launch_fill<float>(rho, g.G, st);
+        launch_deposit<float>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, rho,
+                              st);
+        launch_green<float>(g, o.green_fn, c->dom, c->tw64, c->ghat64, c->ghat32, st);
+        launch_solve<float>(g, rho, static_cast<float2*>(c->S), c->ghat32, c->tw32,
+                            static_cast<float*>(c->phi), c->dom, st);
+        launch_efield<float>(static_cast<float*>(c->phi), g.gx, g.gy, g.gz, c->dom,
+                             static_cast<float4*>(c->E), st);
+        launch_kick<float>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom,
+                           static_cast<float4*>(c->E), b->ref_p, b->mass, st);
+    }
+    HIP_TRY(hipGetLastError());
+    c->applies++;
+    c->last_stream = st;
+    return SC3D_OK;
+}
+
+int sc3d_sync(sc3d_ctx* c)
+{
+    if (!c) return fail(SC3D_ERR_ARG, "sc3d_sync: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->last_stream));
+    int32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, &c->dom->error, sizeof(err), hipMemcpyDeviceToHost));
+    if (err == SC3D_ERR_DOMAIN)
+        return fail(SC3D_ERR_DOMAIN,
+                    "Space_charge_3d_open_hockney_eigen::update_domain: all three spatial "
+                    "dimensions have neglible extent");
+    if (err) return fail(err, "sc3d: device error");
+    return SC3D_OK;
+}
+
+int sc3d_debug_get(sc3d_ctx* c, int what, void* dst, int64_t bytes)
+{
+    if (!c || !dst) return fail(SC3D_ERR_ARG, "sc3d_debug_get: null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->last_stream));
+    const Geometry& g = c->g;
+    auto need = [&](int64_t nb) -> bool { return bytes >= nb; };
+    auto grid_out = [&](const void* src) -> int {
+        if (!need(g.G * 8)) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: buffer too small");
+        if (!c->fp32) {
+            HIP_TRY(hipMemcpy(dst, src, g.G * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(g.G);
+            HIP_TRY(hipMemcpy(tmp.data(), src, g.G * 4, hipMemcpyDeviceToHost));
+            double* d = static_cast<double*>(dst);
+            for (int64_t i = 0; i < g.G; ++i) d[i] = tmp[i];
+        }
+        return SC3D_OK;
+    };
+    switch (what) {
+    case SC3D_DBG_DOMAIN:
+        if (!need(sizeof(DomainRec))) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: too small");
+        HIP_TRY(hipMemcpy(dst, c->dom, sizeof(DomainRec), hipMemcpyDeviceToHost));
+        return SC3D_OK;
+    case SC3D_DBG_KEYS:
+    case SC3D_DBG_PERM: {
+        if (c->sorted_n < 0) return fail(SC3D_ERR_ARG, "sc3d_debug_get: no sort has run");
+        if (!need(c->sorted_n * 4)) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: too small");
+        const int32_t* src = what == SC3D_DBG_KEYS ? c->sb.keys0 : c->sb.perm;
+        HIP_TRY(hipMemcpy(dst, src, c->sorted_n * 4, hipMemcpyDeviceToHost));
+        return SC3D_OK;
+    }
+    case SC3D_DBG_RHO: return grid_out(c->rho);
+    case SC3D_DBG_PHI: return grid_out(c->phi);
+    case SC3D_DBG_EX:
+    case SC3D_DBG_EY:
+    case SC3D_DBG_EZ: {
+        if (!need(g.G * 8)) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: buffer too small");
+        int comp = what - SC3D_DBG_EX;
+        double* d = static_cast<double*>(dst);
+        if (!c->fp32) {
+            std::vector<double> tmp(g.G * 4);
+            HIP_TRY(hipMemcpy(tmp.data(), c->E, g.G * 32, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < g.G; ++i) d[i] = tmp[4 * i + comp];
+        } else {
+            std::vector<float> tmp(g.G * 4);
+            HIP_TRY(hipMemcpy(tmp.data(), c->E, g.G * 16, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < g.G; ++i) d[i] = tmp[4 * i + comp];
+        }
+        return SC3D_OK;
+    }
+    case SC3D_DBG_GHAT:
+        if (!need(g.Ghat_elems * 8)) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: too small");
+        HIP_TRY(hipMemcpy(dst, c->ghat64, g.Ghat_elems * 8, hipMemcpyDeviceToHost));
+        return SC3D_OK;
+    default: return fail(SC3D_ERR_ARG, "sc3d_debug_get: unknown tap");
+    }
+}
+
+int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,
+                           const double mean[6], const double chol[21], uint64_t seed,
+                           int64_t id0, void* stream)
+{
+    if (n < 0 || stride < n || (n > 0 && (!parts || !masks)) || !mean || !chol)
+        return fail(SC3D_ERR_ARG, "sc3d_populate_gaussian: bad argument");
+    if (n == 0) return SC3D_OK;
+    PopulateArgs a{};
+    std::memcpy(a.mean, mean, sizeof(a.mean));
+    std::memcpy(a.chol, chol, sizeof(a.chol));
+    a.seed = seed;
+    a.id0 = id0;
+    launch_populate(parts, stride, n, masks, a, static_cast<hipStream_t>(stream));
+    HIP_TRY(hipGetLastError());
+    return SC3D_OK;
+}
+
+}  // extern "C"

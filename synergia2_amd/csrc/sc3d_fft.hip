@@ -1,0 +1,593 @@
+// sc3d_fft.hip -- the doubled-grid Hockney convolution on MI355X.
+//
+// Replaces the three unpruned 3-D FFTs + complex multiply of the reference
+// (Distributed_fft3d::transform / inv_transform over (Dz, Dy, Dx) and
+// alg_cplx_multiplier, space_charge_3d_open_hockney.cc:641-676) and the
+// Green's function evaluation + its FFT (alg_g2_linear / alg_g2_pointlike,
+// :136-348, 591-639) with hand-written LDS Stockham passes:
+//
+//   x  : R2C of each non-zero rho row (y<gy, z<gz): real row of Dx = 2gx with
+//        only x<gx non-zero, packed as gx complex, one length-gx FFT and the
+//        real-split post-process                        -> S[z][y][kx]
+//   y  : forward length-Dy FFT of each (z<gz, kx) column, rows y>=gy are
+//        zero and are never read                        -> S (all Dy rows)
+//   z  : per (ky, kx) column: forward length-Dz FFT with planes z>=gz zero,
+//        multiply by the real Green's spectrum, inverse FFT, keep z<gz only
+//   y' : inverse length-Dy FFT per (z<gz, kx), keep y<gy
+//   x' : C2R per (z<gz, y<gy) row, keep x<gx             -> phi octant
+//
+// Only the gz planes z<gz of the spectrum are ever stored (S is a quarter
+// of the reference's padded doubled array).  The convention is the
+// reference's (FFTW): forward sign -1, both directions unnormalised, so phi
+// equals the reference's phi2 on the octant x<gx, y<gy, z<gz.
+//
+// The Green's function on the doubled grid is real and even in X, Y and Z
+// (mirror writes of alg_g2_*), so its spectrum is real and even in ky, kz:
+// it is computed from the (gx+1)(gy+1)(gz+1) octant with three real-even
+// passes and stored as that octant only.
+#include <type_traits>
+
+#include "sc3d_internal.h"
+
+namespace sc3d {
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+bool fft_length_supported(int L)
+{
+    return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
+}
+
+// Lines per block: at least 8 so that strided passes move >=128 B runs, and
+// at least 1024 elements so every thread owns one radix-4 butterfly.
+template <int L> struct Lines {
+    static constexpr int value = (1024 / L) > 8 ? (1024 / L) : 8;
+};
+
+template <int R, bool INV, typename T2>
+__device__ __forceinline__ void dft_small(T2* v)
+{
+    if constexpr (R == 2) {
+        T2 a = v[0], b = v[1];
+        v[0].x = a.x + b.x; v[0].y = a.y + b.y;
+        v[1].x = a.x - b.x; v[1].y = a.y - b.y;
+    } else {
+        T2 a, b, c, d;
+        a.x = v[0].x + v[2].x; a.y = v[0].y + v[2].y;
+        b.x = v[0].x - v[2].x; b.y = v[0].y - v[2].y;
+        c.x = v[1].x + v[3].x; c.y = v[1].y + v[3].y;
+        d.x = v[1].x - v[3].x; d.y = v[1].y - v[3].y;
+        v[0].x = a.x + c.x; v[0].y = a.y + c.y;
+        v[2].x = a.x - c.x; v[2].y = a.y - c.y;
+        if (!INV) {  // y1 = b - i d, y3 = b + i d
+            v[1].x = b.x + d.y; v[1].y = b.y - d.x;
+            v[3].x = b.x - d.y; v[3].y = b.y + d.x;
+        } else {     // y1 = b + i d, y3 = b - i d
+            v[1].x = b.x - d.y; v[1].y = b.y + d.x;
+            v[3].x = b.x + d.y; v[3].y = b.y - d.x;
+        }
+    }
+}
+
+// One Stockham autosort pass of radix R on NL lines of length L in LDS
+// (Govindaraju et al., SC'08 formulation): butterfly j reads j + r*L/R,
+// twiddles by w_L^{r*(j mod Ns)*L/(Ns*R)}, writes (j/Ns)*Ns*R + j mod Ns +
+// r*Ns.  All reads of the pass complete before any write (in place).
+template <typename T2, int R, int L, int NL, bool INV>
+__device__ __forceinline__ void stockham_pass(T2* buf, const T2* __restrict__ tw,
+                                              int lns)
+{
+    constexpr int NB = L / R;
+    constexpr int TOT = NB * NL;
+    constexpr int BPT = (TOT + kThreads - 1) / kThreads;
+    constexpr int LL = ilog2(L), LR = ilog2(R);
+    const int Ns = 1 << lns;
+    T2 v[BPT][R];
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        int b = threadIdx.x + q * kThreads;
+        if (TOT % kThreads == 0 || b < TOT) {
+            int line = b / NB, j = b % NB;
+            const T2* lb = buf + line * (L + 1);
+            int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = lb[j + r * NB];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                T2 w = tw[(r * k) << (LL - lns - LR)];
+                if (INV) w.y = -w.y;
+                v[q][r] = cmul(v[q][r], w);
+            }
+            dft_small<R, INV>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        int b = threadIdx.x + q * kThreads;
+        if (TOT % kThreads == 0 || b < TOT) {
+            int line = b / NB, j = b % NB;
+            T2* lb = buf + line * (L + 1);
+            int k = j & (Ns - 1);
+            int d = ((j >> lns) << (lns + LR)) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lb[d + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// NL unnormalised length-L FFTs in LDS; forward sign -1 (INV: +1).  The
+// caller has synchronised after filling buf.
+template <typename T2, int L, int NL, bool INV>
+__device__ __forceinline__ void lds_fft(T2* buf, const T2* __restrict__ tw)
+{
+    constexpr int LOG = ilog2(L);
+    int lns = 0;
+#pragma unroll
+    for (int p = 0; p < LOG / 2; ++p, lns += 2)
+        stockham_pass<T2, 4, L, NL, INV>(buf, tw, lns);
+    if constexpr (LOG & 1) stockham_pass<T2, 2, L, NL, INV>(buf, tw, lns);
+}
+
+// ---------------------------------------------------------------------------
+// x pass, forward: real rows of the rho octant -> half spectrum rows of S.
+// rows r in [0, gz*gy); input row r is rho + r*gx (gx = M reals); the
+// doubled row has Dx = 2M reals, so packed complex z[n] = (x[2n], x[2n+1])
+// is non-zero only for n < M/2.
+// ---------------------------------------------------------------------------
+template <typename R, int M>
+__global__ __launch_bounds__(kThreads) void k_x_r2c(const R* __restrict__ rho,
+                                                    typename cplx<R>::type* __restrict__ S,
+                                                    const typename cplx<R>::type* __restrict__ twM,
+                                                    const typename cplx<R>::type* __restrict__ tw2M,
+                                                    int nrows, int gy, int Dy, const DomainRec* dom)
+{
+    using T2 = typename cplx<R>::type;
+    constexpr int NL = Lines<M>::value;
+    __shared__ T2 buf[NL * (M + 1)];
+    if (dom->error) return;
+    const int r0 = blockIdx.x * NL;
+    const int Hx = M + 1;
+    // load: NL consecutive rows are NL*M contiguous reals = NL*M/2 complex
+    const T2* src = reinterpret_cast<const T2*>(rho + int64_t(r0) * M);
+    for (int e = threadIdx.x; e < NL * M; e += kThreads) {
+        int line = e / M, n = e % M;
+        T2 v;
+        if (n < M / 2 && r0 + line < nrows) {
+            v = src[line * (M / 2) + n];
+        } else {
+            v.x = R(0); v.y = R(0);
+        }
+        buf[line * (M + 1) + n] = v;
+    }
+    __syncthreads();
+    lds_fft<T2, M, NL, false>(buf, twM);
+    // post-process: X[k] = 1/2 (A + (-i) w^k B), A = Z[k] + conj Z[M-k],
+    // B = Z[k] - conj Z[M-k], w = e^{-2 pi i / 2M}; k = 0..M.
+    for (int e = threadIdx.x; e < NL * Hx; e += kThreads) {
+        int line = e / Hx, k = e % Hx;
+        int r = r0 + line;
+        if (r >= nrows) continue;
+        T2 zk = buf[line * (M + 1) + (k == M ? 0 : k)];
+        T2 zm = buf[line * (M + 1) + (k == 0 ? 0 : M - k)];
+        T2 A, B;
+        A.x = zk.x + zm.x; A.y = zk.y - zm.y;
+        B.x = zk.x - zm.x; B.y = zk.y + zm.y;
+        T2 Cc = cmul(tw2M[k], B);
+        T2 X;
+        X.x = R(0.5) * (A.x + Cc.y);
+        X.y = R(0.5) * (A.y - Cc.x);
+        int z = r / gy, y = r - z * gy;
+        S[(int64_t(z) * Dy + y) * Hx + k] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x pass, inverse (C2R): rows r of S (z<gz, y<gy) -> phi octant rows (gx=M).
+// The imaginary parts of the DC and Nyquist terms are ignored (FFTW c2r).
+// Z'[k] = A + i conj(w^k) B, A = X[k] + conj X[M-k], B = X[k] - conj X[M-k];
+// z = inverse FFT_M(Z'); x[2n] = Re z[n], x[2n+1] = Im z[n]; keep n < M/2.
+// ---------------------------------------------------------------------------
+template <typename R, int M>
+__global__ __launch_bounds__(kThreads) void k_x_c2r(const typename cplx<R>::type* __restrict__ S,
+                                                    R* __restrict__ phi,
+                                                    const typename cplx<R>::type* __restrict__ twM,
+                                                    const typename cplx<R>::type* __restrict__ tw2M,
+                                                    int nrows, int gy, int Dy, const DomainRec* dom)
+{
+    using T2 = typename cplx<R>::type;
+    constexpr int NL = Lines<M>::value;
+    __shared__ T2 buf[NL * (M + 1)];
+    if (dom->error) return;
+    const int r0 = blockIdx.x * NL;
+    const int Hx = M + 1;
+    for (int e = threadIdx.x; e < NL * M; e += kThreads) {
+        int line = e / M, k = e % M;
+        int r = r0 + line;
+        T2 v;
+        if (r < nrows) {
+            int z = r / gy, y = r - z * gy;
+            const T2* row = S + (int64_t(z) * Dy + y) * Hx;
+            T2 xk = row[k];
+            T2 xm = row[M - k];
+            if (k == 0) { xk.y = R(0); xm.y = R(0); }
+            T2 A, B;
+            A.x = xk.x + xm.x; A.y = xk.y - xm.y;
+            B.x = xk.x - xm.x; B.y = xk.y + xm.y;
+            T2 Cc = cmul_conj(B, tw2M[k]);  // conj(w^k) * B
+            v.x = A.x - Cc.y;
+            v.y = A.y + Cc.x;
+        } else {
+            v.x = R(0); v.y = R(0);
+        }
+        buf[line * (M + 1) + k] = v;
+    }
+    __syncthreads();
+    lds_fft<T2, M, NL, true>(buf, twM);
+    T2* dst = reinterpret_cast<T2*>(phi + int64_t(r0) * M);
+    for (int e = threadIdx.x; e < NL * (M / 2); e += kThreads) {
+        int line = e / (M / 2), n = e % (M / 2);
+        if (r0 + line < nrows) dst[line * (M / 2) + n] = buf[line * (M + 1) + n];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column passes over S.  Line l -> (plane = l / lpp, c = l % lpp), element p
+// at S[plane*plane_stride + p*pstride + c].  Loads p < nload (rest zero),
+// stores p < nstore.  MODE 0: forward; 1: inverse; 2: forward, multiply by
+// the real Green's spectrum, inverse (the z pass).
+// ---------------------------------------------------------------------------
+template <typename R, int L, int MODE>
+__global__ __launch_bounds__(kThreads) void k_cols(typename cplx<R>::type* __restrict__ S,
+                                                   const typename cplx<R>::type* __restrict__ tw,
+                                                   const R* __restrict__ ghat, int nlines, int lpp,
+                                                   int64_t plane_stride, int64_t pstride,
+                                                   int nload, int nstore, int gy, int Hx,
+                                                   const DomainRec* dom)
+{
+    using T2 = typename cplx<R>::type;
+    constexpr int NL = Lines<L>::value;
+    __shared__ T2 buf[NL * (L + 1)];
+    if (dom->error) return;
+    const int l0 = blockIdx.x * NL;
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        T2 v;
+        if (p < nload && l < nlines) {
+            int plane = l / lpp, c = l - plane * lpp;
+            v = S[plane * plane_stride + p * pstride + c];
+        } else {
+            v.x = R(0); v.y = R(0);
+        }
+        buf[line * (L + 1) + p] = v;
+    }
+    __syncthreads();
+    if constexpr (MODE == 1) {
+        lds_fft<T2, L, NL, true>(buf, tw);
+    } else {
+        lds_fft<T2, L, NL, false>(buf, tw);
+    }
+    if constexpr (MODE == 2) {
+        // line l = ky*Hx + kx over the full Dy rows; element p = kz.
+        const int Dy2 = lpp / Hx;  // Dy
+        for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+            int p = e / NL, line = e % NL;
+            int l = l0 + line;
+            if (l >= nlines) continue;
+            int ky = l / Hx, kx = l - ky * Hx;
+            int my = ky <= Dy2 / 2 ? ky : Dy2 - ky;
+            int mz = p <= L / 2 ? p : L - p;
+            R gv = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
+            T2 v = buf[line * (L + 1) + p];
+            v.x = v.x * gv;
+            v.y = v.y * gv;
+            buf[line * (L + 1) + p] = v;
+        }
+        __syncthreads();
+        lds_fft<T2, L, NL, true>(buf, tw);
+    }
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        if (p < nstore && l < nlines) {
+            int plane = l / lpp, c = l - plane * lpp;
+            S[plane * plane_stride + p * pstride + c] = buf[line * (L + 1) + p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Green's functions (space_charge_3d_open_hockney.cc:136-348).  Value at the
+// doubled-grid point (X, Y, Z) is f(x, y, z) with x = min(X, Dx-X)*hx etc.,
+// exactly the value the reference's mirror writes leave there.
+// ---------------------------------------------------------------------------
+__device__ double green_linear(double x, double y, double z, double hz, double G000)
+{
+    double rr = x * x + y * y;
+    double s_rr_0 = sqrt(rr + z * z);
+    double s_rr_n = sqrt(rr + (z - hz) * (z - hz));
+    double s_rr_p = sqrt(rr + (z + hz) * (z + hz));
+    double G = 2.0 * s_rr_0 - s_rr_n - s_rr_p;
+    double T1, T2, r1, r2;
+    const double epsz = 1e-12 * hz;
+    // z >= 0 here, so of the reference's branches (:294-326) only these three
+    // are reachable; the z < -hz and |z + hz| < epsz ones never are.
+    if (fabs(z) < epsz) {
+        if (fabs(x) + fabs(y) < 2. * epsz) {
+            G += hz * G000;
+        } else {
+            r1 = (sqrt(hz * hz + rr) + hz) / sqrt(rr);
+            G += 2.0 * hz * log(r1);
+        }
+    } else if (fabs(z - hz) < epsz) {
+        r1 = (s_rr_p + z + hz) / (s_rr_0 + z);
+        T1 = (hz + z) * log(r1);
+        G += T1;
+    } else if (z > hz) {
+        r1 = (s_rr_0 + z) / (s_rr_n + z - hz);
+        T1 = (hz - z) * log(r1);
+        r2 = (s_rr_p + z + hz) / (s_rr_0 + z);
+        T2 = (hz + z) * log(r2);
+        G += T1 + T2;
+    }
+    return G;
+}
+
+// G1: rows (Z<=gz, Y<=gy) of the doubled grid: generate, R2C along x, keep
+// the real part of kx = 0..M (the row is even, its spectrum real).
+template <int M>
+__global__ __launch_bounds__(kThreads) void k_green_x(double* __restrict__ ghat,
+                                                      const double2* __restrict__ twM,
+                                                      const double2* __restrict__ tw2M,
+                                                      int gy, int gz, int green_fn,
+                                                      const DomainRec* dom)
+{
+    constexpr int NL = Lines<M>::value;
+    constexpr int Dx = 2 * M;
+    __shared__ double2 buf[NL * (M + 1)];
+    __shared__ double row[NL * (M + 1)];
+    if (dom->error || !dom->ghat_dirty) return;
+    const int nrows = (gz + 1) * (gy + 1);
+    const int r0 = blockIdx.x * NL;
+    const double hx = dom->h[0], hy = dom->h[1], hz = dom->h[2];
+    double G000;
+    if (green_fn == SC3D_GREEN_LINEAR) {
+        double rr = hx * hx + hy * hy;
+        double r1 = sqrt(hx * hx + hy * hy + hz * hz);
+        G000 = (2.0 / rr) * (hz * r1 + rr * log((hz + r1) / sqrt(rr)) - hz * hz);
+    } else {
+        G000 = 2.8 / fmin(hx, fmin(hy, hz));
+    }
+    for (int e = threadIdx.x; e < NL * (M + 1); e += kThreads) {
+        int line = e / (M + 1), X = e % (M + 1);
+        int r = r0 + line;
+        double v = 0.0;
+        if (r < nrows) {
+            int Z = r / (gy + 1), Y = r - Z * (gy + 1);
+            double x = X * hx, y = Y * hy, z = Z * hz;
+            if (green_fn == SC3D_GREEN_LINEAR) {
+                v = green_linear(x, y, z, hz, G000);
+            } else {
+                v = (X == 0 && Y == 0 && Z == 0) ? G000 : 1.0 / sqrt(x * x + y * y + z * z);
+            }
+        }
+        row[e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NL * M; e += kThreads) {
+        int line = e / M, n = e % M;
+        int X0 = 2 * n, X1 = 2 * n + 1;
+        int m0 = X0 <= M ? X0 : Dx - X0;
+        int m1 = X1 <= M ? X1 : Dx - X1;
+        double2 v;
+        v.x = row[line * (M + 1) + m0];
+        v.y = row[line * (M + 1) + m1];
+        buf[line * (M + 1) + n] = v;
+    }
+    __syncthreads();
+    lds_fft<double2, M, NL, false>(buf, twM);
+    const int Hx = M + 1;
+    for (int e = threadIdx.x; e < NL * Hx; e += kThreads) {
+        int line = e / Hx, k = e % Hx;
+        int r = r0 + line;
+        if (r >= nrows) continue;
+        double2 zk = buf[line * (M + 1) + (k == M ? 0 : k)];
+        double2 zm = buf[line * (M + 1) + (k == 0 ? 0 : M - k)];
+        double2 A, B;
+        A.x = zk.x + zm.x; A.y = zk.y - zm.y;
+        B.x = zk.x - zm.x; B.y = zk.y + zm.y;
+        double2 Cc = cmul(tw2M[k], B);
+        ghat[int64_t(r) * Hx + k] = 0.5 * (A.x + Cc.y);
+    }
+}
+
+// G2 / G3: real-even column transforms in place over the octant buffer.
+// Line l -> (plane = l / Hx, kx = l % Hx); element p of the doubled axis
+// reads the stored octant index min(p, L-p); writes real parts p <= L/2.
+// AXIS 1 (y): address (plane*(gy+1) + p)*Hx + kx, plane = Z <= gz.
+// AXIS 2 (z): address (p*(gy+1) + plane)*Hx + kx, plane = ky <= gy.
+template <int L, int AXIS>
+__global__ __launch_bounds__(kThreads) void k_green_cols(double* __restrict__ ghat,
+                                                         const double2* __restrict__ tw,
+                                                         int nlines, int gy, int Hx,
+                                                         const DomainRec* dom)
+{
+    constexpr int NL = Lines<L>::value;
+    __shared__ double2 buf[NL * (L + 1)];
+    if (dom->error || !dom->ghat_dirty) return;
+    const int l0 = blockIdx.x * NL;
+    auto addr = [&](int l, int p) -> int64_t {
+        int plane = l / Hx, kx = l - plane * Hx;
+        if (AXIS == 1) return (int64_t(plane) * (gy + 1) + p) * Hx + kx;
+        return (int64_t(p) * (gy + 1) + plane) * Hx + kx;
+    };
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        double2 v;
+        v.y = 0.0;
+        v.x = l < nlines ? ghat[addr(l, p <= L / 2 ? p : L - p)] : 0.0;
+        buf[line * (L + 1) + p] = v;
+    }
+    __syncthreads();
+    lds_fft<double2, L, NL, false>(buf, tw);
+    for (int e = threadIdx.x; e < NL * (L / 2 + 1); e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        if (l < nlines) ghat[addr(l, p)] = buf[line * (L + 1) + p].x;
+    }
+}
+
+template <typename R>
+__global__ void k_cast_ghat(const double* __restrict__ src, R* __restrict__ dst, int64_t n,
+                            const DomainRec* dom)
+{
+    if (dom->error || !dom->ghat_dirty) return;
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = R(src[i]);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch on the runtime lengths
+// ---------------------------------------------------------------------------
+template <template <int> class F, typename... A>
+void dispatch_len(int L, A&&... a)
+{
+    switch (L) {
+    case 8: F<8>::run(a...); break;
+    case 16: F<16>::run(a...); break;
+    case 32: F<32>::run(a...); break;
+    case 64: F<64>::run(a...); break;
+    case 128: F<128>::run(a...); break;
+    case 256: F<256>::run(a...); break;
+    case 512: F<512>::run(a...); break;
+    default: break;
+    }
+}
+
+inline int nblocks(int64_t items, int per) { return int((items + per - 1) / per); }
+
+template <int M> struct GreenX {
+    static void run(const Geometry& g, int green_fn, const DomainRec* dom, const double2* tw,
+                    double* ghat, hipStream_t st)
+    {
+        int rows = (g.gz + 1) * (g.gy + 1);
+        hipLaunchKernelGGL(k_green_x<M>, dim3(nblocks(rows, Lines<M>::value)), dim3(kThreads), 0,
+                           st, ghat, tw + twiddle_offset(M), tw + twiddle_offset(2 * M), g.gy,
+                           g.gz, green_fn, dom);
+    }
+};
+template <int L> struct GreenY {
+    static void run(const Geometry& g, const DomainRec* dom, const double2* tw, double* ghat,
+                    hipStream_t st)
+    {
+        int lines = (g.gz + 1) * g.Hx;
+        hipLaunchKernelGGL((k_green_cols<L, 1>), dim3(nblocks(lines, Lines<L>::value)),
+                           dim3(kThreads), 0, st, ghat, tw + twiddle_offset(L), lines, g.gy,
+                           g.Hx, dom);
+    }
+};
+template <int L> struct GreenZ {
+    static void run(const Geometry& g, const DomainRec* dom, const double2* tw, double* ghat,
+                    hipStream_t st)
+    {
+        int lines = (g.gy + 1) * g.Hx;
+        hipLaunchKernelGGL((k_green_cols<L, 2>), dim3(nblocks(lines, Lines<L>::value)),
+                           dim3(kThreads), 0, st, ghat, tw + twiddle_offset(L), lines, g.gy,
+                           g.Hx, dom);
+    }
+};
+
+template <typename R>
+void launch_green(const Geometry& g, int green_fn, const DomainRec* dom, const double2* tw64,
+                  double* ghat64, R* ghat_r, hipStream_t st)
+{
+    dispatch_len<GreenX>(g.gx, g, green_fn, dom, tw64, ghat64, st);
+    dispatch_len<GreenY>(g.Dy, g, dom, tw64, ghat64, st);
+    dispatch_len<GreenZ>(g.Dz, g, dom, tw64, ghat64, st);
+    if constexpr (!std::is_same<R, double>::value) {
+        hipLaunchKernelGGL(k_cast_ghat<R>, dim3(256), dim3(256), 0, st, ghat64, ghat_r,
+                           g.Ghat_elems, dom);
+    }
+}
+
+template <typename R> struct SolveX {
+    template <int M> struct Fwd {
+        static void run(const Geometry& g, const R* rho, typename cplx<R>::type* S,
+                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        {
+            int rows = g.gz * g.gy;
+            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, Lines<M>::value)),
+                               dim3(kThreads), 0, st, rho, S, tw + twiddle_offset(M),
+                               tw + twiddle_offset(2 * M), rows, g.gy, g.Dy, dom);
+        }
+    };
+    template <int M> struct Inv {
+        static void run(const Geometry& g, const typename cplx<R>::type* S, R* phi,
+                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        {
+            int rows = g.gz * g.gy;
+            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, Lines<M>::value)),
+                               dim3(kThreads), 0, st, S, phi, tw + twiddle_offset(M),
+                               tw + twiddle_offset(2 * M), rows, g.gy, g.Dy, dom);
+        }
+    };
+    template <int L> struct YFwd {
+        static void run(const Geometry& g, typename cplx<R>::type* S,
+                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        {
+            int lines = g.gz * g.Hx;
+            hipLaunchKernelGGL((k_cols<R, L, 0>), dim3(nblocks(lines, Lines<L>::value)),
+                               dim3(kThreads), 0, st, S, tw + twiddle_offset(L),
+                               (const R*)nullptr, lines, g.Hx, int64_t(g.Dy) * g.Hx,
+                               int64_t(g.Hx), g.gy, g.Dy, g.gy, g.Hx, dom);
+        }
+    };
+    template <int L> struct YInv {
+        static void run(const Geometry& g, typename cplx<R>::type* S,
+                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        {
+            int lines = g.gz * g.Hx;
+            hipLaunchKernelGGL((k_cols<R, L, 1>), dim3(nblocks(lines, Lines<L>::value)),
+                               dim3(kThreads), 0, st, S, tw + twiddle_offset(L),
+                               (const R*)nullptr, lines, g.Hx, int64_t(g.Dy) * g.Hx,
+                               int64_t(g.Hx), g.Dy, g.gy, g.gy, g.Hx, dom);
+        }
+    };
+    template <int L> struct ZConv {
+        static void run(const Geometry& g, typename cplx<R>::type* S,
+                        const typename cplx<R>::type* tw, const R* ghat, const DomainRec* dom,
+                        hipStream_t st)
+        {
+            int lines = g.Dy * g.Hx;
+            hipLaunchKernelGGL((k_cols<R, L, 2>), dim3(nblocks(lines, Lines<L>::value)),
+                               dim3(kThreads), 0, st, S, tw + twiddle_offset(L), ghat, lines,
+                               lines, int64_t(0), int64_t(lines), g.gz, g.gz, g.gy, g.Hx, dom);
+        }
+    };
+};
+
+template <typename R>
+void launch_solve(const Geometry& g, const R* rho, typename cplx<R>::type* S, const R* ghat,
+                  const typename cplx<R>::type* tw, R* phi, const DomainRec* dom, hipStream_t st)
+{
+    dispatch_len<SolveX<R>::template Fwd>(g.gx, g, rho, S, tw, dom, st);
+    dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, S, tw, dom, st);
+    dispatch_len<SolveX<R>::template ZConv>(g.Dz, g, S, tw, ghat, dom, st);
+    dispatch_len<SolveX<R>::template YInv>(g.Dy, g, S, tw, dom, st);
+    dispatch_len<SolveX<R>::template Inv>(g.gx, g, S, phi, tw, dom, st);
+}
+
+template void launch_green<double>(const Geometry&, int, const DomainRec*, const double2*,
+                                   double*, double*, hipStream_t);
+template void launch_green<float>(const Geometry&, int, const DomainRec*, const double2*,
+                                  double*, float*, hipStream_t);
+template void launch_solve<double>(const Geometry&, const double*, double2*, const double*,
+                                   const double2*, double*, const DomainRec*, hipStream_t);
+template void launch_solve<float>(const Geometry&, const float*, float2*, const float*,
+                                  const float2*, float*, const DomainRec*, hipStream_t);
+
+}  // namespace sc3d

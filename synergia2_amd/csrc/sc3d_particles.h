@@ -1,0 +1,65 @@
+// sc3d_particles.h -- host launch interface of the particle-side kernels.
+// Internal to libsc3d_hip.so.
+#pragma once
+
+#include "sc3d_internal.h"
+
+namespace sc3d {
+
+template <typename R> struct vec4;
+template <> struct vec4<double> { using type = double4; };
+template <> struct vec4<float> { using type = float4; };
+
+struct DomainFinishArgs {
+    int32_t fixed, green_fn, grid_entire_period, charge;
+    int32_t shape[3];
+    double size[3], offset[3];
+    double n_sigma, z_period, kick_scale;
+    double total_num, real_num;
+    double time_step, ref_p, ref_gamma, ref_beta;
+};
+
+struct PopulateArgs {
+    double mean[6];
+    double chol[21];
+    uint64_t seed;
+    int64_t id0;
+};
+
+struct SortBuffers {
+    int32_t* keys0;   // unsorted keys (kept for inspection)
+    int32_t* keysA;
+    int32_t* keysB;
+    int32_t* valsA;
+    int32_t* valsB;
+    int32_t* counts;  // 256 * tiles
+    int32_t* totals;  // 256
+    int32_t* sorted_keys;  // result views (point into A/B)
+    int32_t* perm;
+};
+
+constexpr int kSortTileItems = 4096;
+
+void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                   double* P, const DomainFinishArgs& a, DomainRec* dom, hipStream_t st);
+int sort_passes(int64_t G);
+void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
+                 int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st);
+void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
+                    double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st);
+template <typename R>
+void launch_fill(R* p, int64_t n, hipStream_t st);
+template <typename R>
+void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
+                    int gy, int gz, const DomainRec* dom, R* rho, hipStream_t st);
+template <typename R>
+void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
+                   typename vec4<R>::type* E, hipStream_t st);
+template <typename R>
+void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
+                 int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
+                 double m, hipStream_t st);
+void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
+                     const PopulateArgs& a, hipStream_t st);
+
+}  // namespace sc3d

@@ -1,0 +1,728 @@
+// sc3d_particles.hip -- particle-side kernels of the MI355X space-charge step:
+// bunch extent, cell keys + stable radix sort + in-place reorder, CIC
+// deposit, E field, gather-interpolate-kick, and the Gaussian populate.
+#include "sc3d_particles.h"
+
+namespace sc3d {
+
+// ---------------------------------------------------------------------------
+// Bunch extent: the six sums of Core_diagnostics::calculate_spatial_mean_stddev
+// (core_diagnostics.cc:139-153) in one fixed tree, identical for every launch
+// configuration and device:
+//   chunk c = particles [4096c, 4096c+4096); lane t adds, from +0.0 and in
+//   order j = 0..15, particle 4096c + 256j + t (masked and >= n skipped);
+//   lanes combine pairwise (s[t] += s[t+w], w = 1..128), first inside each
+//   64-wide wavefront by xor butterflies, then across the four wavefronts;
+//   chunk partials combine by the same rule in k_domain_finish.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void block_pairwise6(double (&s)[6], double (*ws)[6])
+{
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s[q] = s[q] + __shfl_xor(s[q], w, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ws[wave][q] = s[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s[q] = (ws[0][q] + ws[1][q]) + (ws[2][q] + ws[3][q]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_domain_partial(const double* __restrict__ parts,
+                                                             int64_t stride,
+                                                             const uint8_t* __restrict__ masks,
+                                                             int64_t n, double* __restrict__ P)
+{
+    __shared__ double ws[4][6];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t c = blockIdx.x;
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+        int64_t i = c * 4096 + int64_t(j) * 256 + threadIdx.x;
+        if (i < n && masks[i]) {
+            double x = parts[i], y = parts[i + 2 * stride], z = parts[i + 4 * stride];
+            s[0] += x;
+            s[1] += y;
+            s[2] += z;
+            s[3] += x * x;
+            s[4] += y * y;
+            s[5] += z * z;
+        }
+    }
+    block_pairwise6(s, ws);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) P[c * 6 + q] = s[q];
+}
+
+// space_charge_3d_open_hockney.cc:94-109
+__device__ double smallest_non_tiny(double val, double o1, double o2, double tiny)
+{
+    if (val > tiny) return val;
+    if ((o1 > tiny) && (o2 > tiny)) return fmin(o1, o2);
+    return fmax(o1, o2);
+}
+
+// Finishes the domain on the device (update_domain, .cc:493-540, with the
+// host finish of core_diagnostics.cc:454-459 and Rectangular_grid_domain,
+// rectangular_grid_domain.h:20-36) and derives weight0 (deposit.cc:608-610),
+// the force normalisation (.cc:707-735) and the kick factor (.cc:759-770).
+// Also decides whether the cached Green's spectrum is still valid (same h
+// bits) or must be recomputed this step.
+__global__ __launch_bounds__(kThreads) void k_domain_finish(const double* __restrict__ P,
+                                                            int64_t nch, DomainFinishArgs a,
+                                                            DomainRec* __restrict__ dom)
+{
+    __shared__ double ws[4][6];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!a.fixed) {
+        for (int64_t c = threadIdx.x; c < nch; c += kThreads)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s[q] += P[c * 6 + q];
+        block_pairwise6(s, ws);
+    }
+    if (threadIdx.x != 0) return;
+    DomainRec d = *dom;  // keeps the Green's cache fields
+    double size[3], offset[3];
+    d.error = 0;
+    if (a.fixed) {
+        for (int k = 0; k < 3; ++k) {
+            size[k] = a.size[k];
+            offset[k] = a.offset[k];
+            d.sums[k] = d.sums[k + 3] = 0.0;
+            d.mean[k] = d.stddev[k] = 0.0;
+        }
+    } else {
+        for (int q = 0; q < 6; ++q) d.sums[q] = s[q];
+        for (int j = 0; j < 3; ++j) {
+            d.mean[j] = s[j] / a.total_num;
+            d.stddev[j] = sqrt(s[j + 3] / a.total_num - d.mean[j] * d.mean[j]);
+        }
+        const double tiny = 1.0e-10;
+        if ((d.stddev[0] < tiny) && (d.stddev[1] < tiny) && (d.stddev[2] < tiny)) {
+            d.error = SC3D_ERR_DOMAIN;
+            d.ghat_dirty = 0;
+            *dom = d;
+            return;
+        }
+        offset[0] = d.mean[0];
+        offset[1] = d.mean[1];
+        offset[2] = d.mean[2];
+        size[0] = a.n_sigma * smallest_non_tiny(d.stddev[0], d.stddev[1], d.stddev[2], tiny);
+        size[1] = a.n_sigma * smallest_non_tiny(d.stddev[1], d.stddev[0], d.stddev[2], tiny);
+        size[2] = a.n_sigma * smallest_non_tiny(d.stddev[2], d.stddev[0], d.stddev[1], tiny);
+        if (a.grid_entire_period) {
+            offset[2] = 0.0;
+            size[2] = a.z_period;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        d.size[k] = size[k];
+        d.offset[k] = offset[k];
+        d.left[k] = offset[k] - size[k] / 2.0;
+        d.h[k] = size[k] / (1.0 * a.shape[k]);
+        d.inv_h[k] = 1.0 / d.h[k];
+        d.half_inv_h[k] = 0.5 / d.h[k];
+    }
+    const double* h = d.h;
+    d.w0 = (a.real_num / a.total_num) * a.charge * kE / (h[0] * h[1] * h[2]);
+    double norm = h[0] * h[1] * h[2];
+    norm *= 1.0 / (4.0 * kPi * kEpsilon0);
+    norm *= 1.0;
+    if (a.green_fn == SC3D_GREEN_LINEAR) norm *= 1.0 / (h[2] * h[2]);
+    norm *= 1.0 / (2 * a.shape[0] * 2 * a.shape[1] * 2 * a.shape[2]);
+    d.fn_norm = norm;
+    double q = a.charge * kE;
+    double unit_conversion = kC / (1e9 * kE);
+    d.factor = a.kick_scale * unit_conversion * q * a.time_step * d.fn_norm /
+               (a.ref_p * a.ref_gamma * a.ref_gamma * a.ref_beta);
+    bool same = d.ghat_valid && __double_as_longlong(d.ghat_h[0]) == __double_as_longlong(h[0]) &&
+                __double_as_longlong(d.ghat_h[1]) == __double_as_longlong(h[1]) &&
+                __double_as_longlong(d.ghat_h[2]) == __double_as_longlong(h[2]);
+    d.ghat_dirty = same ? 0 : 1;
+    for (int k = 0; k < 3; ++k) d.ghat_h[k] = h[k];
+    d.ghat_valid = 1;
+    *dom = d;
+}
+
+// ---------------------------------------------------------------------------
+// Cell keys: k = (iz*gy + iy)*gx + ix for active particles with lower corner
+// in [0, g-2]^3 (the particles the deposit and the kick touch), else G.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_keys(const double* __restrict__ parts,
+                                                   int64_t stride,
+                                                   const uint8_t* __restrict__ masks,
+                                                   int64_t n, int gx, int gy, int gz,
+                                                   const DomainRec* __restrict__ dom,
+                                                   int32_t* __restrict__ keys)
+{
+    if (dom->error) return;
+    const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
+    const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
+    const int32_t G = gx * gy * gz;
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kThreads) {
+        double o;
+        int ix = cell_index(parts[i], lx, ihx, o);
+        int iy = cell_index(parts[i + 2 * stride], ly, ihy, o);
+        int iz = cell_index(parts[i + 4 * stride], lz, ihz, o);
+        bool in = masks[i] && ix >= 0 && ix < gx - 1 && iy >= 0 && iy < gy - 1 && iz >= 0 &&
+                  iz < gz - 1;
+        keys[i] = in ? (iz * gy + iy) * gx + ix : G;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, index), 8-bit digits, tiles of 4096.
+// ---------------------------------------------------------------------------
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kThreads * kSortItems;
+
+__global__ __launch_bounds__(kThreads) void k_radix_count(const int32_t* __restrict__ keys,
+                                                          int64_t n, int shift,
+                                                          int32_t* __restrict__ counts,
+                                                          int nblk, const DomainRec* dom)
+{
+    __shared__ int32_t hist[256];
+    if (dom->error) return;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = int64_t(blockIdx.x) * kSortTile;
+    for (int j = 0; j < kSortItems; ++j) {
+        int64_t i = base + int64_t(j) * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    counts[int64_t(threadIdx.x) * nblk + blockIdx.x] = hist[threadIdx.x];
+}
+
+// One block per digit: exclusive scan of that digit's per-tile counts (in
+// place) and the digit total.
+__global__ __launch_bounds__(kThreads) void k_radix_scan(int32_t* __restrict__ counts, int nblk,
+                                                         int32_t* __restrict__ totals,
+                                                         const DomainRec* dom)
+{
+    __shared__ int32_t part[kThreads];
+    if (dom->error) return;
+    int32_t* c = counts + int64_t(blockIdx.x) * nblk;
+    const int per = (nblk + kThreads - 1) / kThreads;
+    const int b0 = threadIdx.x * per;
+    int32_t s = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nblk) s += c[b0 + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 1; w < kThreads; w *= 2) {  // inclusive Hillis-Steele scan
+        int32_t v = threadIdx.x >= w ? part[threadIdx.x - w] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int32_t run = part[threadIdx.x] - s;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nblk) {
+            int32_t v = c[b0 + k];
+            c[b0 + k] = run;
+            run += v;
+        }
+    if (threadIdx.x == kThreads - 1) totals[blockIdx.x] = part[kThreads - 1];
+}
+
+// Stable scatter.  Items are ranked in tile order: 16 rounds of 256; inside
+// a round, wavefront w holds items [64w, 64w+64); the rank of an item among
+// equal digits of its wavefront comes from 8 ballots.
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(
+    const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int64_t n,
+    int shift, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals, int nblk,
+    int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, const DomainRec* dom)
+{
+    __shared__ int32_t base_d[256];
+    __shared__ int32_t run_d[256];
+    __shared__ int32_t wave_d[4][256];
+    if (dom->error) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // digit base = exclusive prefix of totals + this tile's scanned count
+    {
+        int32_t v = totals[t];
+        base_d[t] = v;
+        __syncthreads();
+        for (int w = 1; w < 256; w *= 2) {
+            int32_t u = t >= w ? base_d[t - w] : 0;
+            __syncthreads();
+            base_d[t] += u;
+            __syncthreads();
+        }
+        int32_t excl = base_d[t] - v + counts[int64_t(t) * nblk + blockIdx.x];
+        __syncthreads();
+        base_d[t] = excl;
+        run_d[t] = 0;
+    }
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t tile = int64_t(blockIdx.x) * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        wave_d[0][t] = 0;
+        wave_d[1][t] = 0;
+        wave_d[2][t] = 0;
+        wave_d[3][t] = 0;
+        __syncthreads();
+        int64_t i = tile + int64_t(r) * kThreads + t;
+        bool valid = i < n;
+        int32_t key = valid ? keys_in[i] : 0;
+        int32_t val = valid ? (vals_in ? vals_in[i] : int32_t(i)) : 0;
+        int d = (key >> shift) & 255;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t m = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        int rank = __popcll(peers & lt_mask);
+        if (valid && rank == 0) wave_d[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int64_t pos = base_d[d] + run_d[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wave_d[w][d];
+            if (pos >= 0 && pos < n) {  // always true for consistent counts
+                keys_out[pos] = key;
+                vals_out[pos] = val;
+            }
+        }
+        __syncthreads();
+        run_d[t] += wave_d[0][t] + wave_d[1][t] + wave_d[2][t] + wave_d[3][t];
+    }
+}
+
+// Reorder: scratch[j] = bunch[perm[j]] (7 columns + mask), then copy back.
+__global__ __launch_bounds__(kThreads) void k_gather_particles(
+    const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
+    const int32_t* __restrict__ perm, int64_t n, double* __restrict__ scratch,
+    uint8_t* __restrict__ smask, const DomainRec* dom)
+{
+    if (dom->error) return;
+    for (int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x; j < n;
+         j += int64_t(gridDim.x) * kThreads) {
+        int64_t src = perm[j];
+        if (src < 0 || src >= n) src = j;  // perm is a permutation of [0, n)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) scratch[j + c * n] = parts[src + c * stride];
+        smask[j] = masks[src];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_copy_back(double* __restrict__ parts, int64_t stride,
+                                                        uint8_t* __restrict__ masks, int64_t n,
+                                                        const double* __restrict__ scratch,
+                                                        const uint8_t* __restrict__ smask,
+                                                        const DomainRec* dom)
+{
+    if (dom->error) return;
+    for (int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x; j < n;
+         j += int64_t(gridDim.x) * kThreads) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) parts[j + c * stride] = scratch[j + c * n];
+        masks[j] = smask[j];
+    }
+}
+
+template <typename R>
+__global__ void k_fill(R* __restrict__ p, int64_t n, R v)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// CIC deposit (sv_zyx_rho_reducer_non_periodic, deposit.cc:309-395, launched
+// :588-629): 8 corners with weights w0*aox*aoy*aoz (left to right), a corner
+// written only if 0 < idx < g-1 in all three axes (ingrid, :21-30).
+// Target: the G octant of rho2 (zyx, gx fastest).
+//
+// Each thread takes a run of kDepRun consecutive particles and sums the
+// corners of consecutive particles of the same cell in registers; a run is
+// flushed when the cell changes.  When the block's particles fit a small
+// 3-D bounding box (cell-sorted bunch) the flushes go to an LDS tile that is
+// then added to rho with one global atomic per touched node; otherwise they
+// go straight to global atomics.
+// ---------------------------------------------------------------------------
+constexpr int kDepRun = 8;
+constexpr int kDepChunk = kThreads * kDepRun;
+constexpr int kDepTile = 4096;
+
+template <typename R>
+__device__ __forceinline__ void atomic_add_to(R* p, R v)
+{
+    unsafeAtomicAdd(p, v);
+}
+
+template <typename R, bool LDS>
+__device__ __forceinline__ void dep_flush(R* tgt, int ix, int iy, int iz, const double (&acc)[8],
+                                          int gx, int gy, int gz, int bx0, int by0, int bz0,
+                                          int tx, int ty)
+{
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        int cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+        if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1) {
+            int64_t a = LDS ? int64_t((cz - bz0) * ty + (cy - by0)) * tx + (cx - bx0)
+                            : (int64_t(cz) * gy + cy) * gx + cx;
+            atomic_add_to(tgt + a, R(acc[c]));
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__ parts,
+                                                      int64_t stride,
+                                                      const uint8_t* __restrict__ masks,
+                                                      int64_t n, int gx, int gy, int gz,
+                                                      const DomainRec* __restrict__ dom,
+                                                      R* __restrict__ rho)
+{
+    __shared__ R tile[kDepTile];
+    __shared__ int bb[6];
+    if (dom->error) return;
+    const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
+    const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
+    const double w0 = dom->w0;
+    if (threadIdx.x < 6) bb[threadIdx.x] = (threadIdx.x & 1) ? -1 : 0x7fffffff;
+    __syncthreads();
+    const int64_t i0 = int64_t(blockIdx.x) * kDepChunk + int64_t(threadIdx.x) * kDepRun;
+    // pass 1: bounding box of the cells of in-grid particles
+    int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
+    for (int j = 0; j < kDepRun; ++j) {
+        int64_t i = i0 + j;
+        if (i >= n || !masks[i]) continue;
+        double o;
+        int ix = cell_index(parts[i], lx, ihx, o);
+        int iy = cell_index(parts[i + 2 * stride], ly, ihy, o);
+        int iz = cell_index(parts[i + 4 * stride], lz, ihz, o);
+        if (ix < 0 || ix > gx - 2 || iy < 0 || iy > gy - 2 || iz < 0 || iz > gz - 2) continue;
+        mn[0] = min(mn[0], ix); mx[0] = max(mx[0], ix);
+        mn[1] = min(mn[1], iy); mx[1] = max(mx[1], iy);
+        mn[2] = min(mn[2], iz); mx[2] = max(mx[2], iz);
+    }
+    if (mx[0] >= 0) {
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&bb[2 * k], mn[k]);
+            atomicMax(&bb[2 * k + 1], mx[k]);
+        }
+    }
+    __syncthreads();
+    if (bb[1] < 0) return;  // no particle of this block is in the grid
+    const int bx0 = bb[0], by0 = bb[2], bz0 = bb[4];
+    const int tx = bb[1] - bb[0] + 2, ty = bb[3] - bb[2] + 2, tz = bb[5] - bb[4] + 2;
+    const bool use_lds = int64_t(tx) * ty * tz <= kDepTile;
+    if (use_lds) {
+        for (int e = threadIdx.x; e < tx * ty * tz; e += kThreads) tile[e] = R(0);
+        __syncthreads();
+    }
+    // pass 2: runs of same-cell particles
+    double acc[8];
+    int cx = -1, cy = 0, cz = 0;
+    for (int j = 0; j < kDepRun; ++j) {
+        int64_t i = i0 + j;
+        if (i >= n || !masks[i]) continue;
+        double ox, oy, oz;
+        int ix = cell_index(parts[i], lx, ihx, ox);
+        int iy = cell_index(parts[i + 2 * stride], ly, ihy, oy);
+        int iz = cell_index(parts[i + 4 * stride], lz, ihz, oz);
+        if (ix < 0 || ix > gx - 2 || iy < 0 || iy > gy - 2 || iz < 0 || iz > gz - 2) continue;
+        if (ix != cx || iy != cy || iz != cz) {
+            if (cx >= 0) {
+                if (use_lds)
+                    dep_flush<R, true>(tile, cx, cy, cz, acc, gx, gy, gz, bx0, by0, bz0, tx, ty);
+                else
+                    dep_flush<R, false>(rho, cx, cy, cz, acc, gx, gy, gz, 0, 0, 0, 0, 0);
+            }
+            cx = ix; cy = iy; cz = iz;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+        }
+        const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
+        acc[0] += w0 * aox * aoy * aoz;
+        acc[1] += w0 * ox * aoy * aoz;
+        acc[2] += w0 * aox * oy * aoz;
+        acc[3] += w0 * ox * oy * aoz;
+        acc[4] += w0 * aox * aoy * oz;
+        acc[5] += w0 * ox * aoy * oz;
+        acc[6] += w0 * aox * oy * oz;
+        acc[7] += w0 * ox * oy * oz;
+    }
+    if (cx >= 0) {
+        if (use_lds)
+            dep_flush<R, true>(tile, cx, cy, cz, acc, gx, gy, gz, bx0, by0, bz0, tx, ty);
+        else
+            dep_flush<R, false>(rho, cx, cy, cz, acc, gx, gy, gz, 0, 0, 0, 0, 0);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < tx * ty * tz; e += kThreads) {
+            R v = tile[e];
+            if (v != R(0)) {
+                int x = e % tx, yz = e / tx, y = yz % ty, z = yz / ty;
+                atomic_add_to(rho + (int64_t(bz0 + z) * gy + (by0 + y)) * gx + (bx0 + x), v);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E field (zyx::alg_force_extractor_doubled_domain, space_charge_3d_kernels.h:
+// 217-307): E = -(phi[+1] - phi[-1]) * (0.5/h) on interior nodes, 0 on the
+// boundary.  Stored interleaved (Ex, Ey, Ez, 0) per node for the kick's
+// gathers.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_efield(const R* __restrict__ phi, int gx, int gy,
+                                                     int gz, const DomainRec* __restrict__ dom,
+                                                     typename vec4<R>::type* __restrict__ E)
+{
+    if (dom->error) return;
+    const double ihx = dom->half_inv_h[0], ihy = dom->half_inv_h[1], ihz = dom->half_inv_h[2];
+    const int64_t G = int64_t(gx) * gy * gz;
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < G;
+         i += int64_t(gridDim.x) * kThreads) {
+        int ix = int(i % gx);
+        int64_t r = i / gx;
+        int iy = int(r % gy), iz = int(r / gy);
+        typename vec4<R>::type e;
+        e.x = R(0); e.y = R(0); e.z = R(0); e.w = R(0);
+        if (!(ix == 0 || ix == gx - 1 || iy == 0 || iy == gy - 1 || iz == 0 || iz == gz - 1)) {
+            const int64_t sy = gx, sz = int64_t(gx) * gy;
+            e.x = R(-(double(phi[i + 1]) - double(phi[i - 1])) * ihx);
+            e.y = R(-(double(phi[i + sy]) - double(phi[i - sy])) * ihy);
+            e.z = R(-(double(phi[i + sz]) - double(phi[i - sz])) * ihz);
+        }
+        E[i] = e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kick (zyx::alg_kicker, space_charge_3d_kernels.h:91-215): kicked iff
+// 0 <= idx < g-1 in all axes; trilinear Ez -> energy kick on dp/p, then Ey
+// -> y', then Ex -> x', each interpolation summed in the reference's corner
+// order.  Momenta accumulate in FP64 for every precision.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, int64_t stride,
+                                                   const uint8_t* __restrict__ masks, int64_t n,
+                                                   int gx, int gy, int gz,
+                                                   const DomainRec* __restrict__ dom,
+                                                   const typename vec4<R>::type* __restrict__ E,
+                                                   double pref, double m)
+{
+    if (dom->error) return;
+    const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
+    const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
+    const double factor = dom->factor;
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kThreads) {
+        if (!masks[i]) continue;
+        double ox, oy, oz;
+        int ix = cell_index(parts[i], lx, ihx, ox);
+        int iy = cell_index(parts[i + 2 * stride], ly, ihy, oy);
+        int iz = cell_index(parts[i + 4 * stride], lz, ihz, oz);
+        if (!((ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) && (iz >= 0 && iz < gz - 1)))
+            continue;
+        const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
+        const int64_t b0 = (int64_t(iz) * gy + iy) * gx + ix;
+        const int64_t b1 = b0 + int64_t(gx) * gy;
+        typename vec4<R>::type e[8] = {E[b0], E[b0 + 1], E[b0 + gx], E[b0 + gx + 1],
+                                       E[b1], E[b1 + 1], E[b1 + gx], E[b1 + gx + 1]};
+        const double w[8] = {aoz * aoy * aox, aoz * aoy * ox, aoz * oy * aox, aoz * oy * ox,
+                             oz * aoy * aox,  oz * aoy * ox,  oz * oy * aox,  oz * oy * ox};
+        double vz = w[0] * double(e[0].z), vy = w[0] * double(e[0].y), vx = w[0] * double(e[0].x);
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {
+            vz += w[c] * double(e[c].z);
+            vy += w[c] * double(e[c].y);
+            vx += w[c] * double(e[c].x);
+        }
+        double p = pref + parts[i + 5 * stride] * pref;
+        double Eoc_i = sqrt(p * p + m * m);
+        double Eoc_f = Eoc_i + factor * (-pref) * vz;
+        double dpop = (sqrt(Eoc_f * Eoc_f - m * m) - sqrt(Eoc_i * Eoc_i - m * m)) / pref;
+        parts[i + 5 * stride] += dpop;
+        parts[i + 3 * stride] += factor * vy;
+        parts[i + 1 * stride] += factor * vx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Populate: Philox-4x32-10 (Salmon et al., SC'11) keyed by the 64-bit seed,
+// counter (id_lo, id_hi, call, 0); 53-bit uniforms in (0,1); Box-Muller.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b)
+{
+    uint64_t m = (uint64_t(a) << 21) | (b >> 11);
+    return (double(m) + 0.5) * 0x1.0p-53;
+}
+
+__global__ __launch_bounds__(kThreads) void k_populate(double* __restrict__ parts, int64_t stride,
+                                                       int64_t n, uint8_t* __restrict__ masks,
+                                                       PopulateArgs a)
+{
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kThreads) {
+        int64_t id = a.id0 + i;
+        uint2 key = make_uint2(uint32_t(a.seed), uint32_t(a.seed >> 32));
+        double u[6];
+        for (int call = 0; call < 3; ++call) {
+            uint4 r = philox4x32_10(
+                make_uint4(uint32_t(id), uint32_t(uint64_t(id) >> 32), uint32_t(call), 0u), key);
+            u[2 * call] = u53(r.x, r.y);
+            u[2 * call + 1] = u53(r.z, r.w);
+        }
+        double g[6];
+        for (int k = 0; k < 3; ++k) {
+            double rad = sqrt(-2.0 * log(u[2 * k]));
+            double s, c;
+            sincospi(2.0 * u[2 * k + 1], &s, &c);
+            g[2 * k] = rad * c;
+            g[2 * k + 1] = rad * s;
+        }
+        for (int r = 0; r < 6; ++r) {
+            double v = a.mean[r];
+            for (int c = 0; c <= r; ++c) v += a.chol[r * (r + 1) / 2 + c] * g[c];
+            parts[i + r * stride] = v;
+        }
+        parts[i + 6 * stride] = double(id);
+        masks[i] = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static int grid_for(int64_t n, int cap = 8192)
+{
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    return int(b < cap ? b : cap);
+}
+
+void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                   double* P, const DomainFinishArgs& a, DomainRec* dom, hipStream_t st)
+{
+    int64_t nch = (n + 4095) / 4096;
+    if (!a.fixed && nch > 0)
+        hipLaunchKernelGGL(k_domain_partial, dim3(unsigned(nch)), dim3(kThreads), 0, st, parts,
+                           stride, masks, n, P);
+    hipLaunchKernelGGL(k_domain_finish, dim3(1), dim3(kThreads), 0, st, P, a.fixed ? 0 : nch, a,
+                       dom);
+}
+
+int sort_passes(int64_t G)
+{
+    int bits = 0;
+    while ((int64_t(1) << bits) <= G) ++bits;  // keys in [0, G]
+    return (bits + 7) / 8;
+}
+
+void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
+                 int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride, masks, n,
+                       gx, gy, gz, dom, b.keys0);
+    const int nblk = int((n + kSortTile - 1) / kSortTile);
+    const int passes = sort_passes(int64_t(gx) * gy * gz);
+    const int32_t* kin = b.keys0;
+    const int32_t* vin = nullptr;
+    int32_t* kout[2] = {b.keysA, b.keysB};
+    int32_t* vout[2] = {b.valsA, b.valsB};
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_count, dim3(nblk), dim3(kThreads), 0, st, kin, n, shift,
+                           b.counts, nblk, dom);
+        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(kThreads), 0, st, b.counts, nblk,
+                           b.totals, dom);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kThreads), 0, st, kin, vin, n, shift,
+                           b.counts, b.totals, nblk, kout[p & 1], vout[p & 1], dom);
+        kin = kout[p & 1];
+        vin = vout[p & 1];
+    }
+    b.sorted_keys = const_cast<int32_t*>(kin);
+    b.perm = const_cast<int32_t*>(vin);
+}
+
+void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
+                    double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gather_particles, dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
+                       stride, masks, perm, n, scratch, smask, dom);
+    hipLaunchKernelGGL(k_copy_back, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride,
+                       masks, n, scratch, smask, dom);
+}
+
+template <typename R>
+void launch_fill(R* p, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill<R>, dim3(grid_for(n, 2048)), dim3(kThreads), 0, st, p, n, R(0));
+}
+
+template <typename R>
+void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
+                    int gy, int gz, const DomainRec* dom, R* rho, hipStream_t st)
+{
+    int64_t nb = (n + kDepChunk - 1) / kDepChunk;
+    if (nb < 1) return;
+    hipLaunchKernelGGL(k_deposit<R>, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts, stride,
+                       masks, n, gx, gy, gz, dom, rho);
+}
+
+template <typename R>
+void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
+                   typename vec4<R>::type* E, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_efield<R>, dim3(grid_for(int64_t(gx) * gy * gz)), dim3(kThreads), 0, st,
+                       phi, gx, gy, gz, dom, E);
+}
+
+template <typename R>
+void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
+                 int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
+                 double m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kick<R>, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride, masks,
+                       n, gx, gy, gz, dom, E, pref, m);
+}
+
+void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
+                     const PopulateArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_populate, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride, n,
+                       masks, a);
+}
+
+template void launch_fill<double>(double*, int64_t, hipStream_t);
+template void launch_fill<float>(float*, int64_t, hipStream_t);
+template void launch_deposit<double>(const double*, int64_t, const uint8_t*, int64_t, int, int,
+                                     int, const DomainRec*, double*, hipStream_t);
+template void launch_deposit<float>(const double*, int64_t, const uint8_t*, int64_t, int, int,
+                                    int, const DomainRec*, float*, hipStream_t);
+template void launch_efield<double>(const double*, int, int, int, const DomainRec*, double4*,
+                                    hipStream_t);
+template void launch_efield<float>(const float*, int, int, int, const DomainRec*, float4*,
+                                   hipStream_t);
+template void launch_kick<double>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
+                                  const DomainRec*, const double4*, double, double, hipStream_t);
+template void launch_kick<float>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
+                                 const DomainRec*, const float4*, double, double, hipStream_t);
+
+}  // namespace sc3d
