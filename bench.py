@@ -1,0 +1,210 @@
+"""Throughput of the 3D open-Hockney space-charge step on one MI355X.
+
+Workload (the plain FP64 GPU configuration): a proton bunch at 0.8 GeV
+kinetic energy, 4e6 macroparticles, population 1e11, 6D Gaussian matched to
+an O-F-O-D cell (0.5 m drifts, 0.2 m quads k1 = +-0.6 m^-2, rms emittance
+1 mm mrad), sigma_z = 10 mm, sigma_dp/p = 1e-4, seed 1; 4-cell channel with
+10 space-charge kicks per cell (Split_operator_stepper, 40 steps per turn);
+grid 64x64x128 (doubled 128x128x256), adaptive domain, linear Green's
+function, FP64 throughout.
+
+A step = one space-charge step (the hot path: bunch extent, cell sort,
+deposit, Green's spectrum, FFT convolution, E field, kick) of a bunch that is
+tracked through the channel: the half-step drift/quadrupole maps run between
+the kicks as their own kernels.  value = macroparticles x steps / (sum of the
+HIP-event times of the timed steps, first kernel of the step to last, on the
+stream they run on) = particle-kicks/s.  The wall time of the whole timed
+loop, maps included, is reported beside it as the tracking rate.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch over its HIP-event-timed duration, MI355X HBM 8 TB/s) and the CPU
+restatement of the reference operator (oracle/, OpenMP on the host cores)
+timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+CONFIGS = {
+    # name: (n, grid, population, seed, sigma_z [m], sigma_dpop, emit, precision)
+    "fodo_fp64_4M_64x64x128": (4_000_000, (64, 64, 128), 1e11, 1, 0.010, 1e-4, 1e-6, 0),
+    "fodo_fp32grid_16M_128x128x256": (16_000_000, (128, 128, 256), 2e11, 2, 0.020, 1e-4, 1e-6, 1),
+}
+
+
+def algorithmic_bytes(phase, n, g, w):
+    """Algorithmic HBM bytes of one launch (unit: one particle or one node):
+    k_kick   73 B/particle (x,y,cdt,xp,yp,dpop read, xp,yp,dpop written, mask)
+             + 3 field words per node read once;
+    k_deposit 25 B/particle (x,y,cdt, mask) + 1 density word per node written."""
+    G = g[0] * g[1] * g[2]
+    if phase == "sc3d_kick":
+        return 73 * n + 3 * w * G
+    if phase == "sc3d_local_rho":
+        return 25 * n + w * G
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="fodo_fp64_4M_64x64x128", choices=sorted(CONFIGS))
+    ap.add_argument("--sort-period", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import synergia2_amd as sc
+    from synergia2_amd import stepper as S
+
+    n, grid, pop, seed, sigz, sigdp, emit, precision = CONFIGS[args.config]
+    ke = 0.8
+    gamma = (sc.PROTON_MASS + ke) / sc.PROTON_MASS
+    beta, _ = sc.kinematics(sc.PROTON_MASS, gamma)
+    lattice = S.fodo_channel(4)
+    chol = S.matched_chol(S.fodo_channel(1), emit, sigz / beta, sigdp)
+
+    bunch = sc.Bunch(n, pop, gamma=gamma)
+    bunch.populate_gaussian(np.zeros(6), chol, seed=seed)
+    opts = sc.default_options(*grid)
+    opts.precision = precision
+    if args.sort_period is not None:
+        opts.sort_period = args.sort_period
+    op = sc.SpaceCharge3dOpenHockney(opts)
+    prop = S.Propagator(lattice, 40, bunch, op)
+
+    print(f"# {args.config}: n={n} grid={grid} warmup={args.warmup} steps={args.steps} "
+          f"sort_period={opts.sort_period}", file=sys.stderr, flush=True)
+    prop.run(args.warmup, sync=True)
+    torch.cuda.synchronize()
+
+    op.set_timing(True)
+    op.timing(reset=True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    prop.run(args.steps, fuse=False)
+    ev1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    op.sync()
+    wall = t1 - t0
+    ev_ms = ev0.elapsed_time(ev1)
+    phases, napplies = op.timing(reset=True)
+    assert napplies == args.steps
+    sc_s = sum(phases.values()) / 1e3
+    ms_per_step = 1e3 * sc_s / args.steps
+    value = n * args.steps / sc_s
+    print(f"# SC steps {sc_s:.4f}s -> {ms_per_step:.4f} ms/step, {value:.4e} particle-kicks/s; "
+          f"tracking wall {wall:.4f}s (events {ev_ms/1e3:.4f}s) incl. maps -> "
+          f"{n * args.steps / wall:.4e} /s", file=sys.stderr, flush=True)
+    for k, v in phases.items():
+        print(f"#   {k:16s} {v / args.steps * 1e3:9.2f} us/step", file=sys.stderr)
+
+    w = 8 if precision == 0 else 4
+    cands = [(phases[p], p) for p in ("sc3d_kick", "sc3d_local_rho")]
+    dom_ms, dom_phase = max(cands)
+    per_launch_s = dom_ms / args.steps / 1e3
+    B = algorithmic_bytes(dom_phase, n, grid, w)
+    achieved = B / per_launch_s / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": {"sc3d_kick": "k_kick", "sc3d_local_rho": "k_deposit"}[dom_phase],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "traffic": None,
+        "algorithmic_bytes_per_launch": B,
+        "launch_us": round(per_launch_s * 1e6, 2),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(bunch, prop, opts, args.cpu_steps, sc, S)
+
+    out = {
+        "metric": "particle-kicks/s (3D Hockney SC step)",
+        "value": value,
+        "unit": "particle-kicks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == 0 else "f32-grid/f64-particles",
+        "data": "synthetic (seeded 6D Gaussian matched to the FODO cell)",
+        "config": {
+            "workload": args.config.replace("_", " ") + ", FODO 4 cells, 10 kicks/cell",
+            "particles": n, "grid": list(grid), "doubled_grid": [2 * x for x in grid],
+            "green_fn": "linear", "domain": "adaptive", "sort_period": opts.sort_period,
+            "phase_us_per_step": {k: round(v / args.steps * 1e3, 2) for k, v in phases.items()},
+            "tracking_ms_per_step_incl_maps": round(1e3 * wall / args.steps, 4),
+            "tracking_particle_steps_per_s": n * args.steps / wall,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(bunch, prop, opts, steps, sc, S):
+    """The oracle (CPU restatement of the reference operator + maps) on the
+    host cores, same bunch and steps, FP64.  Bounded to a few steps."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cores = min(os.cpu_count() or 1, 16)
+    parts, masks = bunch.host()
+    parts = np.ascontiguousarray(parts)
+    masks = np.ascontiguousarray(masks)
+    b = O.make_bunch(parts, masks, bunch.total_num, bunch.real_num, bunch.charge, bunch.mass,
+                     bunch.gamma, bunch.beta, bunch.p)
+    o = O.make_options(tuple(opts.shape), n_sigma=opts.n_sigma)
+    k0 = prop.i
+    t0 = time.perf_counter()
+    for j in range(steps):
+        k = (k0 + j) % len(prop.steps)
+        first, second, length = prop.steps[k]
+        O.apply_maps(b, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, bunch.p, bunch.mass))
+                         for kd, L, k1 in first])
+        rc, _ = O.apply(o, b, length / (bunch.beta * sc.C_LIGHT), nthreads=cores)
+        assert rc == 0, O.last_error()
+        O.apply_maps(b, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, bunch.p, bunch.mass))
+                         for kd, L, k1 in second])
+    dt = time.perf_counter() - t0
+    return {
+        "value": bunch.n * steps / dt,
+        "unit": "particle-kicks/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{steps} split-operator steps of the same workload ({bunch.n} particles, "
+                  f"grid {tuple(opts.shape)}), oracle/ CPU restatement of the reference "
+                  f"operator with OpenMP ({cores} threads) and its own FFT (FFTW absent); "
+                  f"{dt:.2f} s",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -68,8 +68,10 @@ typedef struct sc3d_options {
     double kick_scale;           /* default 1                                */
     /* build extensions (no reference counterpart) */
     int32_t precision;           /* SC3D_PRECISION_*; default FP64           */
-    int32_t sort_period;         /* sort + reorder the bunch by cell every N
-                                    applies (0 = never); default 1           */
+    int32_t sort_period;         /* the particles are sorted by cell on every
+                                    apply; the bunch storage itself is
+                                    reordered into that order every N applies
+                                    (0 = never); default 1                   */
 } sc3d_options;
 
 typedef struct sc3d_bunch {
@@ -99,7 +101,8 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out);
  * (hipStream_t, NULL = default stream).  Asynchronous: errors found on the
  * device (negligible extent) are reported by sc3d_sync.  When sort_period
  * says so, the first n_active slots of the bunch (all 7 columns and the
- * masks) are reordered in place by cell; particle identity is column 6. */
+ * masks) are reordered in place into cell order; particle identity is
+ * column 6. */
 int sc3d_apply(sc3d_ctx* ctx, const sc3d_bunch* bunch, double time_step,
                void* stream);
 
@@ -125,6 +128,30 @@ const char* sc3d_last_error(void);
                                  /* spectrum, kz<=gz, ky<=gy, kx<=gx         */
 int sc3d_debug_get(sc3d_ctx* ctx, int what, void* host_dst, int64_t bytes);
 
+/* Device-side phase timing of sc3d_apply with HIP events recorded on the
+ * apply stream.  Phase names follow the reference's simple_timer labels
+ * (space_charge_3d_open_hockney.cc:496,545,625,644,757); each phase but
+ * SC3D_PH_SOLVE and SC3D_PH_SORT is exactly one kernel:
+ *   SC3D_PH_DOMAIN  sc3d_domain          (extent partial sums + finish)
+ *   SC3D_PH_SORT    (no counterpart)     (keys, radix passes, reorder)
+ *   SC3D_PH_DEPOSIT sc3d_local_rho       (the deposit kernel only)
+ *   SC3D_PH_GREEN   sc3d_green_fn2_*     (spectrum passes, cheap when cached)
+ *   SC3D_PH_SOLVE   sc3d_local_f         (x, y, z-with-convolution, y', x')
+ *   SC3D_PH_FIELD   get_force            (E field kernel)
+ *   SC3D_PH_KICK    sc3d_kick            (the kick kernel only)
+ * sc3d_get_timing synchronises and returns milliseconds summed over the
+ * applies timed since the last reset, and their count. */
+#define SC3D_PH_DOMAIN 0
+#define SC3D_PH_SORT 1
+#define SC3D_PH_DEPOSIT 2
+#define SC3D_PH_GREEN 3
+#define SC3D_PH_SOLVE 4
+#define SC3D_PH_FIELD 5
+#define SC3D_PH_KICK 6
+#define SC3D_NPHASES 7
+int sc3d_set_timing(sc3d_ctx* ctx, int enable);
+int sc3d_get_timing(sc3d_ctx* ctx, double ms[SC3D_NPHASES], int64_t* napplies, int reset);
+
 /* Domain record copied by SC3D_DBG_DOMAIN (all FP64). */
 typedef struct sc3d_domain_record {
     double sums[6];              /* Sx Sy Scdt Sxx Syy Scdtcdt               */
@@ -136,6 +163,39 @@ typedef struct sc3d_domain_record {
     double ghat_h[3];            /* cell size the cached spectrum is for     */
     int32_t error, ghat_dirty, ghat_valid, pad;
 } sc3d_domain_record;
+
+/* Single-particle maps of a FODO channel, applied on the device as one
+ * fused program (each particle read and written once for all slices).
+ *   SC3D_MAP_DRIFT      <- FF_drift::apply, src/synergia/libFF/ff_drift.h:92-153
+ *                          (drift_unit, libFF/ff_algorithm.h:45-79)
+ *   SC3D_MAP_QUADRUPOLE <- FF_quadrupole::apply, libFF/ff_quadrupole.h:309-565
+ *                          (yoshida6 with thin_quadrupole_unit kicks,
+ *                          ff_algorithm.h:928-936,1835-2008); no offsets,
+ *                          tilt or multipoles.
+ * ref_cdt is the design reference particle's cdt over the slice, from
+ * sc3d_reference_cdt (get_reference_cdt, ff_drift.h:68-87,
+ * ff_quadrupole.h:262-300, for an on-axis reference). */
+#define SC3D_MAP_DRIFT 0
+#define SC3D_MAP_QUADRUPOLE 1
+#define SC3D_MAX_SLICES 16
+typedef struct sc3d_map_slice {
+    int32_t kind;                /* SC3D_MAP_*                               */
+    int32_t yoshida_steps;       /* quadrupole only; reference default 4     */
+    double length;               /* slice length [m]                         */
+    double k1, k1s;              /* quadrupole strengths [m^-2]              */
+    double ref_cdt;              /* design reference cdt over the slice      */
+} sc3d_map_slice;
+
+double sc3d_reference_cdt(const sc3d_map_slice* slice, double ref_p, double mass);
+
+/* Map program that every subsequent sc3d_apply of this context runs on each
+ * active particle right after its kick, in the kick kernel itself (the
+ * independent operators that follow the collective one in a split-operator
+ * step, step.cc:44-78, fused into the same particle pass).  nslices = 0
+ * removes it.  No counterpart in the reference operator. */
+int sc3d_set_post_kick_maps(sc3d_ctx* ctx, const sc3d_map_slice* slices, int nslices);
+int sc3d_apply_maps(const sc3d_bunch* bunch, const sc3d_map_slice* slices, int nslices,
+                    void* stream);
 
 /* Fills slots [0, n) of a bunch: coordinates = mean + L * u with u a
  * standard normal 6-vector from Philox-4x32-10 keyed (seed, id) and

@@ -73,6 +73,11 @@ class Domain(C.Structure):
         return out
 
 
+class Slice(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("yoshida_steps", C.c_int32), ("length", C.c_double),
+                ("k1", C.c_double), ("k1s", C.c_double), ("ref_cdt", C.c_double)]
+
+
 class Debug(C.Structure):
     _fields_ = [
         ("domain", C.POINTER(Domain)),
@@ -117,6 +122,9 @@ def lib():
         L.orc_deposit.argtypes = [C.POINTER(Bunch), C.POINTER(Domain), C.POINTER(C.c_int32), dp, C.c_int]
         L.orc_force.argtypes = [dp, C.POINTER(C.c_int32), dp, dp, dp, dp]
         L.orc_kick.argtypes = [C.POINTER(Bunch), dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(Domain)]
+        L.orc_reference_cdt.argtypes = [C.POINTER(Slice), C.c_double, C.c_double]
+        L.orc_reference_cdt.restype = C.c_double
+        L.orc_apply_maps.argtypes = [C.POINTER(Bunch), C.POINTER(Slice), C.c_int]
         _lib = L
     return _lib
 
@@ -255,3 +263,26 @@ def force(phi2_padded, g, h):
     phi = np.ascontiguousarray(phi2_padded, np.float64)
     lib().orc_force(_dp(phi), _ip(np.array(g, np.int32)), _dp(np.array(h, np.float64)), _dp(ex), _dp(ey), _dp(ez))
     return ex, ey, ez
+
+
+def make_slices(spec):
+    """spec: list of (kind, length, k1, yoshida_steps, ref_cdt)."""
+    arr = (Slice * len(spec))()
+    for i, (kind, length, k1, steps, ref_cdt) in enumerate(spec):
+        arr[i].kind = kind
+        arr[i].length = length
+        arr[i].k1 = k1
+        arr[i].k1s = 0.0
+        arr[i].yoshida_steps = steps
+        arr[i].ref_cdt = ref_cdt
+    return arr
+
+
+def reference_cdt(kind, length, k1, steps, ref_p, mass):
+    s = make_slices([(kind, length, k1, steps, 0.0)])
+    return lib().orc_reference_cdt(s, float(ref_p), float(mass))
+
+
+def apply_maps(bunch, spec):
+    s = make_slices(spec)
+    lib().orc_apply_maps(C.byref(bunch), s, len(spec))

@@ -835,3 +835,95 @@ extern "C" int orc_apply(const orc_options* opt, orc_bunch* b, double time_step,
         return -1;
     }
 }
+
+// ---------------------------------------------------------------------------
+// libFF maps: drift_unit (ff_algorithm.h:45-79), thin_quadrupole_unit
+// (:928-936), yoshida6 (:1835-2008) as used by FF_quadrupole::apply
+// (ff_quadrupole.h:438-468) and the design-reference cdt
+// (ff_drift.h:68-87, ff_quadrupole.h:262-300).
+// ---------------------------------------------------------------------------
+namespace {
+void ff_drift_unit(double& x, double xp, double& y, double yp, double& cdt, double dpop,
+                   double length, double reference_momentum, double m, double reference_cdt)
+{
+    double uni = 1.0;
+    double sig = double((0.0 < length) - (length < 0.0));
+    double dp = dpop + uni;
+    double inv_npz = uni / std::sqrt(dp * dp - xp * xp - yp * yp);
+    double lxpr = xp * length * inv_npz;
+    double lypr = yp * length * inv_npz;
+    double D2 = lxpr * lxpr + length * length + lypr * lypr;
+    double p = dp * reference_momentum;
+    double E2 = p * p + m * m;
+    double ibeta2 = E2 / (p * p);
+    x = x + lxpr;
+    y = y + lypr;
+    cdt = cdt + sig * std::sqrt(D2 * ibeta2) - reference_cdt;
+}
+
+void ff_thin_quad(double x, double& xp, double y, double& yp, const double* kL)
+{
+    xp = xp - kL[0] * x + kL[1] * y;
+    yp = yp + kL[0] * y + kL[1] * x;
+}
+
+void ff_yoshida6(double& x, double& xp, double& y, double& yp, double& cdt, double dpop,
+                 double ref_p, double m, double step_ref_cdt, double step_length,
+                 const double* ss, int steps)
+{
+    const double c1 = 0.79361246386112147294625603763;
+    const double c2 = -0.206276584816439780698721319354;
+    const double c3 = -0.118008867881292655922412133144;
+    const double c4 = 0.236949573653050744373598734219;
+    const double d1 = 1.58722492772224294589251207526;
+    const double d2 = -1.99977809735512250728995471397;
+    const double d3 = -1.82324266348482825773733634154;
+    const double d4 = 2.29714181079092974648453380998;
+    double sub = step_ref_cdt * 0.1;
+    double k1[2] = {d1 * ss[0], d1 * ss[1]}, k2[2] = {d2 * ss[0], d2 * ss[1]};
+    double k3[2] = {d3 * ss[0], d3 * ss[1]}, k4[2] = {d4 * ss[0], d4 * ss[1]};
+    const double cs[10] = {c1, c2, c2, c3, c4, c4, c3, c2, c2, c1};
+    const double* ks[9] = {k1, k2, k1, k3, k4, k3, k1, k2, k1};
+    for (int i = 0; i < steps; ++i) {
+        for (int d = 0; d < 10; ++d) {
+            ff_drift_unit(x, xp, y, yp, cdt, dpop, cs[d] * step_length, ref_p, m, sub);
+            if (d < 9) ff_thin_quad(x, xp, y, yp, ks[d]);
+        }
+    }
+}
+
+void ff_slice(const orc_slice& s, double& x, double& xp, double& y, double& yp, double& cdt,
+              double dpop, double ref_p, double m)
+{
+    if (s.kind == 0) {
+        ff_drift_unit(x, xp, y, yp, cdt, dpop, s.length, ref_p, m, s.ref_cdt);
+    } else {
+        int steps = s.yoshida_steps;
+        double kn[2] = {s.k1 * (s.length / steps), s.k1s * (s.length / steps)};
+        ff_yoshida6(x, xp, y, yp, cdt, dpop, ref_p, m, s.ref_cdt / steps, s.length / steps, kn,
+                    steps);
+    }
+}
+}  // namespace
+
+extern "C" double orc_reference_cdt(const orc_slice* s, double ref_p, double mass)
+{
+    double x = 0, xp = 0, y = 0, yp = 0, cdt = 0;
+    orc_slice z = *s;
+    z.ref_cdt = 0.0;
+    ff_slice(z, x, xp, y, yp, cdt, 0.0, ref_p, mass);
+    return cdt;
+}
+
+extern "C" void orc_apply_maps(orc_bunch* b, const orc_slice* s, int nslices)
+{
+    double* p = b->parts;
+    const int64_t st = b->stride;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < b->n_active; ++i) {
+        if (!b->masks[i]) continue;
+        for (int k = 0; k < nslices; ++k)
+            ff_slice(s[k], p[i], p[i + st], p[i + 2 * st], p[i + 3 * st], p[i + 4 * st],
+                     p[i + 5 * st], b->ref_p, b->mass);
+    }
+}

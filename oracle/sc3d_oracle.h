@@ -112,6 +112,16 @@ void orc_force(const double* phi2, const int32_t g[3], const double h[3],
 void orc_kick(orc_bunch* b, const double* ex, const double* ey,
               const double* ez, const int32_t g[3], const orc_domain* d);
 
+/* Single-particle maps (libFF restatement): kind 0 drift (drift_unit,
+ * ff_algorithm.h:45-79), kind 1 quadrupole (yoshida6 + thin quadrupole
+ * kicks, ff_algorithm.h:928-936,1835-2008, ff_quadrupole.h:438-468). */
+typedef struct orc_slice {
+    int32_t kind, yoshida_steps;
+    double length, k1, k1s, ref_cdt;
+} orc_slice;
+double orc_reference_cdt(const orc_slice* s, double ref_p, double mass);
+void orc_apply_maps(orc_bunch* b, const orc_slice* s, int nslices);
+
 #ifdef __cplusplus
 }
 #endif

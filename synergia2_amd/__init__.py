@@ -110,8 +110,20 @@ class DomainRecord(C.Structure):
         return out
 
 
+class MapSlice(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("yoshida_steps", C.c_int32), ("length", C.c_double),
+                ("k1", C.c_double), ("k1s", C.c_double), ("ref_cdt", C.c_double)]
+
+
+MAP_DRIFT = 0
+MAP_QUADRUPOLE = 1
+MAX_SLICES = 16
+PHASES = ("sc3d_domain", "sc3d_sort", "sc3d_local_rho", "sc3d_green_fn2", "sc3d_local_f",
+          "sc3d_force", "sc3d_kick")
+
 EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
-           "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian")
+           "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
+           "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps")
 
 _lib = None
 
@@ -137,8 +149,16 @@ def load(path=None):
     L.sc3d_populate_gaussian.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.c_uint64, C.c_int64, C.c_void_p]
+    L.sc3d_reference_cdt.argtypes = [C.POINTER(MapSlice), C.c_double, C.c_double]
+    L.sc3d_reference_cdt.restype = C.c_double
+    L.sc3d_apply_maps.argtypes = [C.POINTER(BunchDesc), C.POINTER(MapSlice), C.c_int, C.c_void_p]
+    L.sc3d_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.sc3d_set_post_kick_maps.argtypes = [C.c_void_p, C.POINTER(MapSlice), C.c_int]
+    L.sc3d_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                  C.c_int]
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
-                 "sc3d_populate_gaussian"):
+                 "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
+                 "sc3d_get_timing", "sc3d_set_post_kick_maps"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L
@@ -277,6 +297,24 @@ class SpaceCharge3dOpenHockney:
         _check(load().sc3d_debug_get(self._ctx, what, out.ctypes.data, out.nbytes))
         return out
 
+    def set_post_kick_maps(self, slices):
+        """Fuse a map program (from make_slices, or None) into the kick."""
+        if slices is None:
+            _check(load().sc3d_set_post_kick_maps(self._ctx, None, 0))
+            return
+        arr, n = slices
+        _check(load().sc3d_set_post_kick_maps(self._ctx, arr, n))
+
+    def set_timing(self, enable=True):
+        _check(load().sc3d_set_timing(self._ctx, int(enable)))
+
+    def timing(self, reset=False):
+        """(dict phase -> total ms, number of timed applies)."""
+        ms = (C.c_double * len(PHASES))()
+        n = C.c_int64()
+        _check(load().sc3d_get_timing(self._ctx, ms, C.byref(n), int(reset)))
+        return dict(zip(PHASES, list(ms))), n.value
+
     def close(self):
         if self._ctx:
             load().sc3d_destroy(self._ctx)
@@ -287,3 +325,30 @@ class SpaceCharge3dOpenHockney:
             self.close()
         except Exception:
             pass
+
+
+def make_slices(spec, ref_p, mass):
+    """spec: iterable of (kind, length, k1[, yoshida_steps]) -> MapSlice array
+    with the design reference cdt of every slice filled in."""
+    spec = list(spec)
+    arr = (MapSlice * max(len(spec), 1))()
+    for i, item in enumerate(spec):
+        kind, length, k1 = item[:3]
+        steps = item[3] if len(item) > 3 else 4
+        arr[i].kind = kind
+        arr[i].length = float(length)
+        arr[i].k1 = float(k1)
+        arr[i].k1s = 0.0
+        arr[i].yoshida_steps = int(steps)
+        arr[i].ref_cdt = 0.0
+        arr[i].ref_cdt = load().sc3d_reference_cdt(C.byref(arr[i]), float(ref_p), float(mass))
+    return arr, len(spec)
+
+
+def apply_maps(bunch, slices, stream=None):
+    """Apply a map program (from make_slices) to the bunch on the device."""
+    arr, n = slices
+    if n == 0:
+        return
+    d = bunch.desc()
+    _check(load().sc3d_apply_maps(C.byref(d), arr, n, _stream_ptr(stream)))

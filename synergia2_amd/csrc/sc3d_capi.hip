@@ -61,6 +61,13 @@ struct sc3d_ctx {
     int64_t sorted_n = -1;
     hipStream_t last_stream = nullptr;
     std::vector<void*> owned;
+    // phase timing: one set of SC3D_NPHASES+1 events per timed apply
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double ph_ms[SC3D_NPHASES] = {0};
+    int64_t ph_applies = 0;
+    MapProgram post{};  // maps fused into the kick kernel (post.n == 0: none)
 };
 
 namespace {
@@ -129,9 +136,28 @@ void make_twiddles(std::vector<double2>& t64, std::vector<float2>& t32)
         }
 }
 
+// Records the next pooled event on st (timing on) and returns it.
+hipEvent_t mark(sc3d_ctx* c, hipStream_t st)
+{
+    if (!c->timing) return nullptr;
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    hipEvent_t e = c->ev_pool[c->ev_used++];
+    (void)hipEventRecord(e, st);
+    return e;
+}
+
 }  // namespace
 
 extern "C" {
+
+__attribute__((visibility("hidden"))) int sc3d_internal_fail(int code, const char* msg)
+{
+    return fail(code, msg);
+}
 
 const char* sc3d_last_error(void) { return g_err.c_str(); }
 
@@ -217,6 +243,7 @@ int sc3d_destroy(sc3d_ctx* c)
 {
     if (!c) return SC3D_OK;
     if (c->last_stream || c->applies) (void)hipStreamSynchronize(c->last_stream);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void* q : c->owned)
         if (q) (void)hipFree(q);
     delete c;
@@ -270,42 +297,71 @@ int sc3d_apply(sc3d_ctx* c, const sc3d_bunch* b, double time_step, void* stream)
     a.ref_p = b->ref_p;
     a.ref_gamma = b->ref_gamma;
     a.ref_beta = b->ref_beta;
+    // the cell sort runs on every apply (the deposit walks the particles in
+    // cell order); the bunch storage is reordered every sort_period applies
+    const bool do_sort = n > 0;
+    const bool do_reorder =
+        do_sort && o.sort_period > 0 && (c->applies % o.sort_period) == 0;
+    if (do_sort)
+        if (int rc = ensure_particle_buffers(c, n)) return rc;
+    MapProgram post = c->post;
+    post.ref_p = b->ref_p;
+    post.mass = b->mass;
+    hipEvent_t ev[SC3D_NPHASES + 1];
+    ev[0] = mark(c, st);
     launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, st);
+    ev[1] = mark(c, st);
 
     // cell sort + in-place reorder of the bunch
-    if (o.sort_period > 0 && (c->applies % o.sort_period) == 0 && n > 0) {
-        if (int rc = ensure_particle_buffers(c, n)) return rc;
+    const int32_t* dep_perm = nullptr;
+    if (do_sort) {
         launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb, st);
-        launch_reorder(b->parts, b->stride, b->masks, n, c->sb.perm, c->scratch, c->smask,
-                       c->dom, st);
         c->sorted_n = n;
+        if (do_reorder)
+            launch_reorder(b->parts, b->stride, b->masks, n, c->sb.perm, c->scratch, c->smask,
+                           c->dom, st);
+        else
+            dep_perm = c->sb.perm;
     }
+    if (!c->fp32)
+        launch_fill<double>(static_cast<double*>(c->rho), g.G, st);
+    else
+        launch_fill<float>(static_cast<float*>(c->rho), g.G, st);
+    ev[2] = mark(c, st);
 
     if (!c->fp32) {
         double* rho = static_cast<double*>(c->rho);
-        launch_fill<double>(rho, g.G, st);
-        launch_deposit<double>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, rho,
-                               st);
+        launch_deposit<double>(b->parts, b->stride, b->masks, dep_perm, n, g.gx, g.gy, g.gz,
+                               c->dom, rho, st);
+        ev[3] = mark(c, st);
         launch_green<double>(g, o.green_fn, c->dom, c->tw64, c->ghat64, c->ghat64, st);
+        ev[4] = mark(c, st);
         launch_solve<double>(g, rho, static_cast<double2*>(c->S), c->ghat64, c->tw64,
                              static_cast<double*>(c->phi), c->dom, st);
+        ev[5] = mark(c, st);
         launch_efield<double>(static_cast<double*>(c->phi), g.gx, g.gy, g.gz, c->dom,
                               static_cast<double4*>(c->E), st);
+        ev[6] = mark(c, st);
         launch_kick<double>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom,
-                            static_cast<double4*>(c->E), b->ref_p, b->mass, st);
+                            static_cast<double4*>(c->E), b->ref_p, b->mass, post, st);
     } else {
         float* rho = static_cast<float*>(c->rho);
-        launch_fill<float>(rho, g.G, st);
-        launch_deposit<float>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, rho,
-                              st);
+        launch_deposit<float>(b->parts, b->stride, b->masks, dep_perm, n, g.gx, g.gy, g.gz,
+                              c->dom, rho, st);
+        ev[3] = mark(c, st);
         launch_green<float>(g, o.green_fn, c->dom, c->tw64, c->ghat64, c->ghat32, st);
+        ev[4] = mark(c, st);
         launch_solve<float>(g, rho, static_cast<float2*>(c->S), c->ghat32, c->tw32,
                             static_cast<float*>(c->phi), c->dom, st);
+        ev[5] = mark(c, st);
         launch_efield<float>(static_cast<float*>(c->phi), g.gx, g.gy, g.gz, c->dom,
                              static_cast<float4*>(c->E), st);
+        ev[6] = mark(c, st);
         launch_kick<float>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom,
-                           static_cast<float4*>(c->E), b->ref_p, b->mass, st);
+                           static_cast<float4*>(c->E), b->ref_p, b->mass, post, st);
     }
+    ev[7] = mark(c, st);
+    (void)ev;
     HIP_TRY(hipGetLastError());
     c->applies++;
     c->last_stream = st;
@@ -384,6 +440,52 @@ int sc3d_debug_get(sc3d_ctx* c, int what, void* dst, int64_t bytes)
         return SC3D_OK;
     default: return fail(SC3D_ERR_ARG, "sc3d_debug_get: unknown tap");
     }
+}
+
+int sc3d_set_post_kick_maps(sc3d_ctx* c, const sc3d_map_slice* slices, int nslices)
+{
+    if (!c || nslices < 0 || nslices > SC3D_MAX_SLICES || (nslices > 0 && !slices))
+        return fail(SC3D_ERR_ARG, "sc3d_set_post_kick_maps: bad argument");
+    for (int k = 0; k < nslices; ++k) {
+        if (slices[k].kind != SC3D_MAP_DRIFT && slices[k].kind != SC3D_MAP_QUADRUPOLE)
+            return fail(SC3D_ERR_ARG, "sc3d_set_post_kick_maps: unknown slice kind");
+        if (slices[k].kind == SC3D_MAP_QUADRUPOLE && slices[k].yoshida_steps < 1)
+            return fail(SC3D_ERR_ARG, "sc3d_set_post_kick_maps: yoshida_steps < 1");
+        c->post.s[k] = slices[k];
+    }
+    c->post.n = nslices;
+    return SC3D_OK;
+}
+
+int sc3d_set_timing(sc3d_ctx* c, int enable)
+{
+    if (!c) return fail(SC3D_ERR_ARG, "sc3d_set_timing: null context");
+    c->timing = enable != 0;
+    return SC3D_OK;
+}
+
+int sc3d_get_timing(sc3d_ctx* c, double ms[SC3D_NPHASES], int64_t* napplies, int reset)
+{
+    if (!c || !ms) return fail(SC3D_ERR_ARG, "sc3d_get_timing: null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->last_stream));
+    const size_t per = SC3D_NPHASES + 1;
+    for (size_t k = 0; k + per <= c->ev_used; k += per) {
+        for (int p = 0; p < SC3D_NPHASES; ++p) {
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, c->ev_pool[k + p], c->ev_pool[k + p + 1]));
+            c->ph_ms[p] += t;
+        }
+        c->ph_applies++;
+    }
+    c->ev_used = 0;
+    for (int p = 0; p < SC3D_NPHASES; ++p) ms[p] = c->ph_ms[p];
+    if (napplies) *napplies = c->ph_applies;
+    if (reset) {
+        for (int p = 0; p < SC3D_NPHASES; ++p) c->ph_ms[p] = 0.0;
+        c->ph_applies = 0;
+    }
+    return SC3D_OK;
 }
 
 int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,

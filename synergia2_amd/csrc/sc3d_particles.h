@@ -3,6 +3,7 @@
 #pragma once
 
 #include "sc3d_internal.h"
+#include "sc3d_maps.h"
 
 namespace sc3d {
 
@@ -50,15 +51,16 @@ void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, co
 template <typename R>
 void launch_fill(R* p, int64_t n, hipStream_t st);
 template <typename R>
-void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
-                    int gy, int gz, const DomainRec* dom, R* rho, hipStream_t st);
+void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
+                    const int32_t* perm, int64_t n, int gx, int gy, int gz, const DomainRec* dom,
+                    R* rho, hipStream_t st);
 template <typename R>
 void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
                    typename vec4<R>::type* E, hipStream_t st);
 template <typename R>
 void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
                  int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
-                 double m, hipStream_t st);
+                 double m, const MapProgram& prog, hipStream_t st);
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                      const PopulateArgs& a, hipStream_t st);
 

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kThreads) void k_domain_partial(const double* __res
     __shared__ double ws[4][6];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int64_t c = blockIdx.x;
-#pragma unroll 4
+#pragma unroll
     for (int j = 0; j < 16; ++j) {
         int64_t i = c * 4096 + int64_t(j) * 256 + threadIdx.x;
         if (i < n && masks[i]) {
@@ -340,131 +340,87 @@ __global__ void k_fill(R* __restrict__ p, int64_t n, R v)
 // written only if 0 < idx < g-1 in all three axes (ingrid, :21-30).
 // Target: the G octant of rho2 (zyx, gx fastest).
 //
-// Each thread takes a run of kDepRun consecutive particles and sums the
-// corners of consecutive particles of the same cell in registers; a run is
-// flushed when the cell changes.  When the block's particles fit a small
-// 3-D bounding box (cell-sorted bunch) the flushes go to an LDS tile that is
-// then added to rho with one global atomic per touched node; otherwise they
-// go straight to global atomics.
+// Global FP64 atomics are the scarce resource on this device, so the eight
+// corner weights are first summed over runs of equal cell keys inside each
+// 64-wide wavefront (a segmented inclusive scan over lanes = consecutive
+// particles; segments are maximal runs of equal keys, so any particle order
+// is correct and a cell-sorted bunch collapses to about one flush per cell
+// and wavefront), and only the last lane of each run issues the atomics.
+// Lanes walk the particles in cell-sorted order, through the permutation of
+// this step's sort (the bunch storage itself is reordered only every
+// sort_period applies, which keeps these gathers nearly coalesced).
 // ---------------------------------------------------------------------------
-constexpr int kDepRun = 8;
-constexpr int kDepChunk = kThreads * kDepRun;
-constexpr int kDepTile = 4096;
-
-template <typename R>
-__device__ __forceinline__ void atomic_add_to(R* p, R v)
-{
-    unsafeAtomicAdd(p, v);
-}
-
-template <typename R, bool LDS>
-__device__ __forceinline__ void dep_flush(R* tgt, int ix, int iy, int iz, const double (&acc)[8],
-                                          int gx, int gy, int gz, int bx0, int by0, int bz0,
-                                          int tx, int ty)
-{
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        int cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
-        if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1) {
-            int64_t a = LDS ? int64_t((cz - bz0) * ty + (cy - by0)) * tx + (cx - bx0)
-                            : (int64_t(cz) * gy + cy) * gx + cx;
-            atomic_add_to(tgt + a, R(acc[c]));
-        }
-    }
-}
-
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__ parts,
                                                       int64_t stride,
                                                       const uint8_t* __restrict__ masks,
+                                                      const int32_t* __restrict__ perm,
                                                       int64_t n, int gx, int gy, int gz,
                                                       const DomainRec* __restrict__ dom,
                                                       R* __restrict__ rho)
 {
-    __shared__ R tile[kDepTile];
-    __shared__ int bb[6];
     if (dom->error) return;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
     const double w0 = dom->w0;
-    if (threadIdx.x < 6) bb[threadIdx.x] = (threadIdx.x & 1) ? -1 : 0x7fffffff;
-    __syncthreads();
-    const int64_t i0 = int64_t(blockIdx.x) * kDepChunk + int64_t(threadIdx.x) * kDepRun;
-    // pass 1: bounding box of the cells of in-grid particles
-    int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
-    for (int j = 0; j < kDepRun; ++j) {
-        int64_t i = i0 + j;
-        if (i >= n || !masks[i]) continue;
-        double o;
-        int ix = cell_index(parts[i], lx, ihx, o);
-        int iy = cell_index(parts[i + 2 * stride], ly, ihy, o);
-        int iz = cell_index(parts[i + 4 * stride], lz, ihz, o);
-        if (ix < 0 || ix > gx - 2 || iy < 0 || iy > gy - 2 || iz < 0 || iz > gz - 2) continue;
-        mn[0] = min(mn[0], ix); mx[0] = max(mx[0], ix);
-        mn[1] = min(mn[1], iy); mx[1] = max(mx[1], iy);
-        mn[2] = min(mn[2], iz); mx[2] = max(mx[2], iz);
+    const int lane = threadIdx.x & 63;
+    const uint64_t le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
+    // lanes take consecutive particles of the cell-sorted order: through the
+    // sort permutation, or the bunch itself when it was just reordered
+    const int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+    int64_t i = j;
+    if (perm && j < n) {
+        i = perm[j];
+        if (i < 0 || i >= n) i = j;  // perm is a permutation of [0, n)
     }
-    if (mx[0] >= 0) {
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&bb[2 * k], mn[k]);
-            atomicMax(&bb[2 * k + 1], mx[k]);
-        }
+    int ix = 0, iy = 0, iz = 0;
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    bool ok = false;
+    if (j < n && masks[i]) {
+        ix = cell_index(parts[i], lx, ihx, ox);
+        iy = cell_index(parts[i + 2 * stride], ly, ihy, oy);
+        iz = cell_index(parts[i + 4 * stride], lz, ihz, oz);
+        ok = ix >= 0 && ix <= gx - 2 && iy >= 0 && iy <= gy - 2 && iz >= 0 && iz <= gz - 2;
     }
-    __syncthreads();
-    if (bb[1] < 0) return;  // no particle of this block is in the grid
-    const int bx0 = bb[0], by0 = bb[2], bz0 = bb[4];
-    const int tx = bb[1] - bb[0] + 2, ty = bb[3] - bb[2] + 2, tz = bb[5] - bb[4] + 2;
-    const bool use_lds = int64_t(tx) * ty * tz <= kDepTile;
-    if (use_lds) {
-        for (int e = threadIdx.x; e < tx * ty * tz; e += kThreads) tile[e] = R(0);
-        __syncthreads();
-    }
-    // pass 2: runs of same-cell particles
-    double acc[8];
-    int cx = -1, cy = 0, cz = 0;
-    for (int j = 0; j < kDepRun; ++j) {
-        int64_t i = i0 + j;
-        if (i >= n || !masks[i]) continue;
-        double ox, oy, oz;
-        int ix = cell_index(parts[i], lx, ihx, ox);
-        int iy = cell_index(parts[i + 2 * stride], ly, ihy, oy);
-        int iz = cell_index(parts[i + 4 * stride], lz, ihz, oz);
-        if (ix < 0 || ix > gx - 2 || iy < 0 || iy > gy - 2 || iz < 0 || iz > gz - 2) continue;
-        if (ix != cx || iy != cy || iz != cz) {
-            if (cx >= 0) {
-                if (use_lds)
-                    dep_flush<R, true>(tile, cx, cy, cz, acc, gx, gy, gz, bx0, by0, bz0, tx, ty);
-                else
-                    dep_flush<R, false>(rho, cx, cy, cz, acc, gx, gy, gz, 0, 0, 0, 0, 0);
-            }
-            cx = ix; cy = iy; cz = iz;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) acc[c] = 0.0;
-        }
+    const int key = ok ? (iz * gy + iy) * gx + ix : -1 - lane;  // lanes out: singletons
+    double w[8];
+    if (ok) {
         const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
-        acc[0] += w0 * aox * aoy * aoz;
-        acc[1] += w0 * ox * aoy * aoz;
-        acc[2] += w0 * aox * oy * aoz;
-        acc[3] += w0 * ox * oy * aoz;
-        acc[4] += w0 * aox * aoy * oz;
-        acc[5] += w0 * ox * aoy * oz;
-        acc[6] += w0 * aox * oy * oz;
-        acc[7] += w0 * ox * oy * oz;
+        w[0] = w0 * aox * aoy * aoz;
+        w[1] = w0 * ox * aoy * aoz;
+        w[2] = w0 * aox * oy * aoz;
+        w[3] = w0 * ox * oy * aoz;
+        w[4] = w0 * aox * aoy * oz;
+        w[5] = w0 * ox * aoy * oz;
+        w[6] = w0 * aox * oy * oz;
+        w[7] = w0 * ox * oy * oz;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w[c] = 0.0;
     }
-    if (cx >= 0) {
-        if (use_lds)
-            dep_flush<R, true>(tile, cx, cy, cz, acc, gx, gy, gz, bx0, by0, bz0, tx, ty);
-        else
-            dep_flush<R, false>(rho, cx, cy, cz, acc, gx, gy, gz, 0, 0, 0, 0, 0);
+    const int kprev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    const int seg = __popcll(__ballot(head) & le_mask);
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        // every lane takes part in every shuffle (no short-circuit): a lane
+        // shuffle must never run under a divergent exec mask
+        const int seg_up = __shfl_up(seg, s, 64);
+        const bool same = (lane >= s) & (seg_up == seg);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const double v = __shfl_up(w[c], s, 64);
+            if (same) w[c] += v;
+        }
     }
-    if (use_lds) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < tx * ty * tz; e += kThreads) {
-            R v = tile[e];
-            if (v != R(0)) {
-                int x = e % tx, yz = e / tx, y = yz % ty, z = yz / ty;
-                atomic_add_to(rho + (int64_t(bz0 + z) * gy + (by0 + y)) * gx + (bx0 + x), v);
-            }
+    const int seg_down = __shfl_down(seg, 1, 64);
+    const bool tail = (lane == 63) | (seg_down != seg);
+    if (ok && tail) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1)
+                unsafeAtomicAdd(rho + (int64_t(cz) * gy + cy) * gx + cx, R(w[c]));
         }
     }
 }
@@ -505,14 +461,17 @@ __global__ __launch_bounds__(kThreads) void k_efield(const R* __restrict__ phi, 
 // 0 <= idx < g-1 in all axes; trilinear Ez -> energy kick on dp/p, then Ey
 // -> y', then Ex -> x', each interpolation summed in the reference's corner
 // order.  Momenta accumulate in FP64 for every precision.
+// MAPS: the map program that follows the kick in the split-operator
+// sequence is applied to every active particle in the same pass (one read
+// and one write of the six coordinates for kick + maps).
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, bool MAPS>
 __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, int64_t stride,
                                                    const uint8_t* __restrict__ masks, int64_t n,
                                                    int gx, int gy, int gz,
                                                    const DomainRec* __restrict__ dom,
                                                    const typename vec4<R>::type* __restrict__ E,
-                                                   double pref, double m)
+                                                   double pref, double m, MapProgram prog)
 {
     if (dom->error) return;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
@@ -521,33 +480,48 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
     for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kThreads) {
         if (!masks[i]) continue;
+        double x = parts[i], y = parts[i + 2 * stride], cdt = parts[i + 4 * stride];
         double ox, oy, oz;
-        int ix = cell_index(parts[i], lx, ihx, ox);
-        int iy = cell_index(parts[i + 2 * stride], ly, ihy, oy);
-        int iz = cell_index(parts[i + 4 * stride], lz, ihz, oz);
-        if (!((ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) && (iz >= 0 && iz < gz - 1)))
-            continue;
-        const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
-        const int64_t b0 = (int64_t(iz) * gy + iy) * gx + ix;
-        const int64_t b1 = b0 + int64_t(gx) * gy;
-        typename vec4<R>::type e[8] = {E[b0], E[b0 + 1], E[b0 + gx], E[b0 + gx + 1],
-                                       E[b1], E[b1 + 1], E[b1 + gx], E[b1 + gx + 1]};
-        const double w[8] = {aoz * aoy * aox, aoz * aoy * ox, aoz * oy * aox, aoz * oy * ox,
-                             oz * aoy * aox,  oz * aoy * ox,  oz * oy * aox,  oz * oy * ox};
-        double vz = w[0] * double(e[0].z), vy = w[0] * double(e[0].y), vx = w[0] * double(e[0].x);
+        const int ix = cell_index(x, lx, ihx, ox);
+        const int iy = cell_index(y, ly, ihy, oy);
+        const int iz = cell_index(cdt, lz, ihz, oz);
+        const bool in = (ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) &&
+                        (iz >= 0 && iz < gz - 1);
+        if (!MAPS && !in) continue;
+        double xp = parts[i + stride], yp = parts[i + 3 * stride], dpop = parts[i + 5 * stride];
+        if (in) {
+            const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
+            const int64_t b0 = (int64_t(iz) * gy + iy) * gx + ix;
+            const int64_t b1 = b0 + int64_t(gx) * gy;
+            typename vec4<R>::type e[8] = {E[b0],  E[b0 + 1],  E[b0 + gx],  E[b0 + gx + 1],
+                                           E[b1],  E[b1 + 1],  E[b1 + gx],  E[b1 + gx + 1]};
+            const double w[8] = {aoz * aoy * aox, aoz * aoy * ox, aoz * oy * aox, aoz * oy * ox,
+                                 oz * aoy * aox,  oz * aoy * ox,  oz * oy * aox,  oz * oy * ox};
+            double vz = w[0] * double(e[0].z), vy = w[0] * double(e[0].y),
+                   vx = w[0] * double(e[0].x);
 #pragma unroll
-        for (int c = 1; c < 8; ++c) {
-            vz += w[c] * double(e[c].z);
-            vy += w[c] * double(e[c].y);
-            vx += w[c] * double(e[c].x);
+            for (int c = 1; c < 8; ++c) {
+                vz += w[c] * double(e[c].z);
+                vy += w[c] * double(e[c].y);
+                vx += w[c] * double(e[c].x);
+            }
+            const double p = pref + dpop * pref;
+            const double Eoc_i = sqrt(p * p + m * m);
+            const double Eoc_f = Eoc_i + factor * (-pref) * vz;
+            dpop += (sqrt(Eoc_f * Eoc_f - m * m) - sqrt(Eoc_i * Eoc_i - m * m)) / pref;
+            yp += factor * vy;
+            xp += factor * vx;
         }
-        double p = pref + parts[i + 5 * stride] * pref;
-        double Eoc_i = sqrt(p * p + m * m);
-        double Eoc_f = Eoc_i + factor * (-pref) * vz;
-        double dpop = (sqrt(Eoc_f * Eoc_f - m * m) - sqrt(Eoc_i * Eoc_i - m * m)) / pref;
-        parts[i + 5 * stride] += dpop;
-        parts[i + 3 * stride] += factor * vy;
-        parts[i + 1 * stride] += factor * vx;
+        if (MAPS) {
+            for (int k = 0; k < prog.n; ++k)
+                apply_slice(prog.s[k], x, xp, y, yp, cdt, dpop, prog.ref_p, prog.mass);
+            parts[i] = x;
+            parts[i + 2 * stride] = y;
+            parts[i + 4 * stride] = cdt;
+        }
+        parts[i + stride] = xp;
+        parts[i + 3 * stride] = yp;
+        parts[i + 5 * stride] = dpop;
     }
 }
 
@@ -677,13 +651,14 @@ void launch_fill(R* p, int64_t n, hipStream_t st)
 }
 
 template <typename R>
-void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
-                    int gy, int gz, const DomainRec* dom, R* rho, hipStream_t st)
+void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
+                    const int32_t* perm, int64_t n, int gx, int gy, int gz, const DomainRec* dom,
+                    R* rho, hipStream_t st)
 {
-    int64_t nb = (n + kDepChunk - 1) / kDepChunk;
+    int64_t nb = (n + kThreads - 1) / kThreads;
     if (nb < 1) return;
     hipLaunchKernelGGL(k_deposit<R>, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts, stride,
-                       masks, n, gx, gy, gz, dom, rho);
+                       masks, perm, n, gx, gy, gz, dom, rho);
 }
 
 template <typename R>
@@ -697,10 +672,14 @@ void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
 template <typename R>
 void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
                  int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
-                 double m, hipStream_t st)
+                 double m, const MapProgram& prog, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_kick<R>, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride, masks,
-                       n, gx, gy, gz, dom, E, pref, m);
+    if (prog.n > 0)
+        hipLaunchKernelGGL((k_kick<R, true>), dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
+                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog);
+    else
+        hipLaunchKernelGGL((k_kick<R, false>), dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
+                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog);
 }
 
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
@@ -712,17 +691,21 @@ void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
 
 template void launch_fill<double>(double*, int64_t, hipStream_t);
 template void launch_fill<float>(float*, int64_t, hipStream_t);
-template void launch_deposit<double>(const double*, int64_t, const uint8_t*, int64_t, int, int,
-                                     int, const DomainRec*, double*, hipStream_t);
-template void launch_deposit<float>(const double*, int64_t, const uint8_t*, int64_t, int, int,
-                                    int, const DomainRec*, float*, hipStream_t);
+template void launch_deposit<double>(const double*, int64_t, const uint8_t*, const int32_t*,
+                                     int64_t, int, int, int, const DomainRec*, double*,
+                                     hipStream_t);
+template void launch_deposit<float>(const double*, int64_t, const uint8_t*, const int32_t*,
+                                    int64_t, int, int, int, const DomainRec*, float*,
+                                    hipStream_t);
 template void launch_efield<double>(const double*, int, int, int, const DomainRec*, double4*,
                                     hipStream_t);
 template void launch_efield<float>(const float*, int, int, int, const DomainRec*, float4*,
                                    hipStream_t);
 template void launch_kick<double>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
-                                  const DomainRec*, const double4*, double, double, hipStream_t);
+                                  const DomainRec*, const double4*, double, double,
+                                  const MapProgram&, hipStream_t);
 template void launch_kick<float>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
-                                 const DomainRec*, const float4*, double, double, hipStream_t);
+                                 const DomainRec*, const float4*, double, double,
+                                 const MapProgram&, hipStream_t);
 
 }  // namespace sc3d

@@ -275,3 +275,102 @@ def test_fp32_grid_field_accuracy(gpu):
             ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez"))]
     print("fp32 field rel-L2", errs, "phi", rel_l2(op.debug(sc.DBG_PHI), out["phi"]))
     assert max(errs) <= 1e-5
+
+
+def test_maps_parity(gpu):
+    """Fused drift/quadrupole program vs the oracle's libFF restatement:
+    bit-exact (same IEEE operations, sqrt and division correctly rounded)."""
+    import oracle as O
+    from synergia2_amd import stepper as S
+    n = 50000
+    b = sc.Bunch(n, 1e11, gamma=GAMMA)
+    chol = S.matched_chol(S.fodo_channel(1), 1e-6, 0.01 / BETA, 1e-3)
+    b.populate_gaussian(np.zeros(6), chol, seed=3)
+    p0, m0 = b.host()
+    spec = [(sc.MAP_DRIFT, 0.37, 0.0), (sc.MAP_QUADRUPOLE, 0.2, 0.6),
+            (sc.MAP_DRIFT, 0.5, 0.0), (sc.MAP_QUADRUPOLE, 0.13, -0.6),
+            (sc.MAP_QUADRUPOLE, 0.07, -0.6)]
+    slices = sc.make_slices(spec, b.p, b.mass)
+    for i, (kd, L, k1) in enumerate(spec):
+        assert slices[0][i].ref_cdt == O.reference_cdt(kd, L, k1, 4, b.p, b.mass)
+    sc.apply_maps(b, slices)
+    pg, _ = b.host()
+    po = np.ascontiguousarray(p0.copy())
+    ob = O.make_bunch(po, np.ascontiguousarray(m0.copy()), n, 1e11, 1, b.mass, b.gamma, b.beta, b.p)
+    O.apply_maps(ob, [(kd, L, k1, 4, slices[0][i].ref_cdt) for i, (kd, L, k1) in enumerate(spec)])
+    for c in range(7):
+        assert np.array_equal(pg[c], po[c]), c
+
+
+def test_fused_kick_maps_equal_unfused(gpu):
+    """Kick with the post-kick map program fused into the kick kernel gives
+    the same bits as the kick followed by a separate map pass."""
+    from synergia2_amd import stepper as S
+    n = 40000
+    chol = S.matched_chol(S.fodo_channel(1), 1e-6, 0.01 / BETA, 1e-4)
+    prog_spec = [(sc.MAP_DRIFT, 0.05, 0.0), (sc.MAP_QUADRUPOLE, 0.06, 0.6),
+                 (sc.MAP_DRIFT, 0.03, 0.0)]
+    outs = []
+    for fused in (True, False):
+        b = sc.Bunch(n, 1e11, gamma=GAMMA)
+        b.populate_gaussian(np.zeros(6), chol, seed=8)
+        op = sc.SpaceCharge3dOpenHockney(make_opts((32, 32, 64)))
+        prog = sc.make_slices(prog_spec, b.p, b.mass)
+        if fused:
+            op.set_post_kick_maps(prog)
+            op.apply_bunch(b, 1e-9)
+        else:
+            op.apply_bunch(b, 1e-9)
+            sc.apply_maps(b, prog)
+            torch_sync()
+        outs.append(b.host()[0])
+    # rho atomics order may differ between the two runs (last-bit level);
+    # dp/p has an absolute floor of ~2e-16 from the energy-kick difference
+    # sqrt(Ef^2-m^2) - sqrt(Ei^2-m^2) (space_charge_3d_kernels.h:170-177)
+    a, c = outs
+    for col in range(7):
+        scale = max(np.abs(c[col]).max(), 1e-300)
+        assert np.abs(a[col] - c[col]).max() <= 1e-12 * scale, col
+
+
+def torch_sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def test_propagation_parity(gpu):
+    """Six split-operator steps through the FODO channel (fused maps, sort
+    every second kick) vs the oracle running the same sequence."""
+    import oracle as O
+    from synergia2_amd import stepper as S
+    n = 30000
+    shape = (32, 32, 64)
+    lattice = S.fodo_channel(4)
+    chol = S.matched_chol(S.fodo_channel(1), 1e-6, 0.01 / BETA, 1e-4)
+    b = sc.Bunch(n, 1e11, gamma=GAMMA)
+    b.populate_gaussian(np.zeros(6), chol, seed=9)
+    p0, m0 = b.host()
+    op = sc.SpaceCharge3dOpenHockney(make_opts(shape, sort_period=2))
+    prop = S.Propagator(lattice, 40, b, op)
+    prop.run(6, sync=True)
+    pg, _ = b.host()
+    # oracle: identical step sequence
+    po = np.ascontiguousarray(p0.copy())
+    mo = np.ascontiguousarray(m0.copy())
+    ob = O.make_bunch(po, mo, n, 1e11, 1, b.mass, b.gamma, b.beta, b.p)
+    for k in range(6):
+        first, second, length = prop.steps[k]
+        O.apply_maps(ob, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, b.p, b.mass))
+                          for kd, L, k1 in first])
+        o = O.make_options(shape, reorder=(k % 2 == 0))
+        rc, _ = O.apply(o, ob, length / (b.beta * sc.C_LIGHT), nthreads=4)
+        assert rc == 0
+        O.apply_maps(ob, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, b.p, b.mass))
+                          for kd, L, k1 in second])
+    g = by_id(pg)
+    r = po[:, np.argsort(po[6], kind="stable")]
+    assert np.array_equal(g[6], r[6])
+    for col in range(6):
+        scale = np.abs(r[col]).max()
+        err = np.abs(g[col] - r[col]).max() / scale
+        assert err <= 1e-12, (col, err)

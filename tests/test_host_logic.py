@@ -1,0 +1,78 @@
+"""Host-side logic that needs no GPU: the C ABI library loads and exports
+every entry point include/sc3d_capi.h declares; option defaults mirror the
+reference; the split-operator slicing and the matched FODO beam."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import synergia2_amd as sc
+from synergia2_amd import stepper as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "sc3d_capi.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sc3d_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_capi_exports_every_declared_symbol():
+    lib = sc.load()
+    names = declared_functions()
+    assert len(names) >= 12
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(sc.EXPORTS)
+
+
+def test_option_defaults_mirror_reference():
+    # implemented_collective_options.h:82-95
+    o = sc.default_options(32, 32, 64)
+    assert list(o.shape) == [32, 32, 64]
+    assert o.green_fn == sc.GREEN_LINEAR
+    assert (o.periodic_z, o.grid_entire_period, o.domain_fixed) == (0, 0, 0)
+    assert (o.z_period, o.n_sigma, o.kick_scale, o.comm_group_size) == (0.0, 8.0, 1.0, 4)
+    sc.set_fixed_domain(o, (1, 2, 3), (4, 5, 6))
+    assert o.domain_fixed == 1 and list(o.offset) == [1, 2, 3] and list(o.size) == [4, 5, 6]
+
+
+def test_reference_cdt_host_function():
+    s = sc.make_slices([(sc.MAP_DRIFT, 0.5, 0.0)], 1.46329603074703, sc.PROTON_MASS)
+    beta = 1.46329603074703 / math.sqrt(1.46329603074703 ** 2 + sc.PROTON_MASS ** 2)
+    assert s[0][0].ref_cdt == pytest.approx(0.5 / beta, rel=1e-15)
+
+
+def test_split_operator_slicing():
+    lat = S.fodo_channel(4)
+    steps = S.split_operator_steps(lat, 40)
+    assert len(steps) == 40
+    total = sum(sum(s[1] for s in a) + sum(s[1] for s in b) for a, b, _ in steps)
+    assert total == pytest.approx(5.6, rel=1e-12)
+    for a, b, length in steps:
+        assert sum(s[1] for s in a) == pytest.approx(length / 2, rel=1e-12)
+        assert sum(s[1] for s in b) == pytest.approx(length / 2, rel=1e-12)
+        assert len(a) + len(b) <= sc.MAX_SLICES
+
+
+def test_matched_fodo_beam():
+    cell = S.fodo_channel(1)
+    bx, ax, mux = S.periodic_twiss(S.cell_matrix(cell, 0))
+    by, ay, muy = S.periodic_twiss(S.cell_matrix(cell, 1))
+    assert 17 < bx < 20 and 17 < by < 20 and 1.0 < abs(ax) < 1.2
+    assert math.degrees(mux) == pytest.approx(4.33, abs=0.01)
+    L = S.matched_chol(cell, 1e-6, 0.012, 1e-4)
+    Lm = np.zeros((6, 6))
+    for r in range(6):
+        for c in range(r + 1):
+            Lm[r, c] = L[r * (r + 1) // 2 + c]
+    cov = Lm @ Lm.T
+    assert math.sqrt(np.linalg.det(cov[:2, :2])) == pytest.approx(1e-6, rel=1e-12)
+    assert cov[0, 0] / 1e-6 == pytest.approx(bx, rel=1e-12)
+    assert -cov[0, 1] / 1e-6 == pytest.approx(ax, rel=1e-12)
+    # one period maps the matched ellipse onto itself
+    M = S.cell_matrix(cell, 0)
+    np.testing.assert_allclose(M @ cov[:2, :2] @ M.T, cov[:2, :2], rtol=1e-10)

@@ -1,0 +1,60 @@
+"""Cell index (collective/utils.h:9-24) and the build-side stable cell sort,
+pinned by integer closed forms on hand-built coordinates and numpy's stable
+argsort."""
+import numpy as np
+
+import oracle as O
+from helpers import BETA, GAMMA, PREF
+
+
+def test_cell_index_closed_forms():
+    left, h = -0.5, 0.25
+    ih = 1.0 / h
+    # cell centres map to idx = k - 1/2 ... s = (pos-left)/h - 0.5
+    for k in range(-3, 8):
+        pos = left + (k + 0.5) * h          # s = k exactly (h is a power of 2)
+        idx, off = O.cell_index(pos, left, ih)
+        assert (idx, off) == (k, 0.0)
+        pos = left + (k + 0.75) * h          # s = k + 0.25
+        idx, off = O.cell_index(pos, left, ih)
+        assert (idx, off) == (k, 0.25)
+    # far outside and NaN: outside every grid, defined result
+    assert O.cell_index(1e300, 0.0, 1.0)[0] < -10 ** 8
+    assert O.cell_index(float("nan"), 0.0, 1.0)[0] < -10 ** 8
+    # negative side floors toward -inf
+    assert O.cell_index(left, left, ih) == (-1, 0.5)
+
+
+def test_keys_and_stable_sort():
+    g = (4, 4, 4)
+    h = 0.25
+    pts = [  # (x, y, z) -> expected key; domain left = -0.5 in each axis
+        ((-0.5 + 0.5 * h, -0.5 + 0.5 * h, -0.5 + 0.5 * h), 0),
+        ((-0.5 + 1.7 * h, -0.5 + 2.9 * h, -0.5 + 1.2 * h), (0 * 4 + 2) * 4 + 1),
+        ((-0.5 + 3.6 * h, 0.0, 0.0), 64),      # ix = 3 = g-1 -> not in [0, g-2]
+        ((-0.5 + 0.2 * h, 0.0, 0.0), 64),      # ix = -1
+        ((-0.5 + 2.5 * h, -0.5 + 1.5 * h, -0.5 + 2.5 * h), (2 * 4 + 1) * 4 + 2),
+        ((-0.5 + 1.7 * h, -0.5 + 2.9 * h, -0.5 + 1.2 * h), 64),  # masked out
+    ]
+    n = len(pts)
+    parts = np.zeros((7, n))
+    for i, (p, _) in enumerate(pts):
+        parts[0, i], parts[2, i], parts[4, i] = p
+    masks = np.ones(n, np.uint8)
+    masks[5] = 0
+    b = O.make_bunch(parts, masks, n, 1e10, 1, 0.938272046, GAMMA, BETA, PREF)
+    dom = O.Domain()
+    for a in range(3):
+        dom.left[a] = -0.5
+        dom.h[a] = h
+    keys = np.zeros(n, np.int32)
+    import ctypes as C
+    O.lib().orc_cell_keys(C.byref(b), C.byref(dom), np.array(g, np.int32).ctypes.data_as(
+        C.POINTER(C.c_int32)), keys.ctypes.data_as(C.POINTER(C.c_int32)))
+    assert list(keys) == [k for _, k in pts]
+    rng = np.random.default_rng(0)
+    k = rng.integers(0, 50, 10000).astype(np.int32)
+    perm = np.zeros(10000, np.int32)
+    O.lib().orc_stable_sort(k.ctypes.data_as(C.POINTER(C.c_int32)), 10000,
+                            perm.ctypes.data_as(C.POINTER(C.c_int32)))
+    assert np.array_equal(perm, np.argsort(k, kind="stable"))
