@@ -71,7 +71,7 @@ typedef struct sc3d_options {
     int32_t sort_period;         /* the particles are sorted by cell on every
                                     apply; the bunch storage itself is
                                     reordered into that order every N applies
-                                    (0 = never); default 1                   */
+                                    (0 = never); default 8                   */
 } sc3d_options;
 
 typedef struct sc3d_bunch {
@@ -196,6 +196,16 @@ double sc3d_reference_cdt(const sc3d_map_slice* slice, double ref_p, double mass
 int sc3d_set_post_kick_maps(sc3d_ctx* ctx, const sc3d_map_slice* slices, int nslices);
 int sc3d_apply_maps(const sc3d_bunch* bunch, const sc3d_map_slice* slices, int nslices,
                     void* stream);
+
+/* Moment diagnostics of the bunch on the device: mean[6] and the
+ * symmetric 6x6 second moments about the mean, both divided by total_num
+ * (Core_diagnostics::calculate_mean / calculate_mom2,
+ * src/synergia/bunch/core_diagnostics.cc:202-229,318-362), summed with the
+ * same fixed tree as the bunch extent.  Synchronises `stream`.  rms
+ * emittances follow as sqrt(det) of the 2x2 blocks (calculate_emittances,
+ * diagnostics_full2_host.cc:20-40). */
+int sc3d_moments(sc3d_ctx* ctx, const sc3d_bunch* bunch, double mean[6], double mom2[36],
+                 void* stream);
 
 /* Fills slots [0, n) of a bunch: coordinates = mean + L * u with u a
  * standard normal 6-vector from Philox-4x32-10 keyed (seed, id) and

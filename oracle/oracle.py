@@ -125,6 +125,7 @@ def lib():
         L.orc_reference_cdt.argtypes = [C.POINTER(Slice), C.c_double, C.c_double]
         L.orc_reference_cdt.restype = C.c_double
         L.orc_apply_maps.argtypes = [C.POINTER(Bunch), C.POINTER(Slice), C.c_int]
+        L.orc_moments.argtypes = [dp, C.c_int64, C.c_int64, C.POINTER(C.c_uint8), C.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -286,3 +287,14 @@ def reference_cdt(kind, length, k1, steps, ref_p, mass):
 def apply_maps(bunch, spec):
     s = make_slices(spec)
     lib().orc_apply_maps(C.byref(bunch), s, len(spec))
+
+
+def moments(parts, masks, total_num=None):
+    """(mean[6], mom2[6,6]) with the fixed tree."""
+    parts = np.ascontiguousarray(parts)
+    n = masks.shape[0]
+    mean = np.zeros(6)
+    mom2 = np.zeros(36)
+    lib().orc_moments(_dp(parts), parts.shape[1], n, masks.ctypes.data_as(C.POINTER(C.c_uint8)),
+                      float(total_num if total_num is not None else n), _dp(mean), _dp(mom2))
+    return mean, mom2.reshape(6, 6)

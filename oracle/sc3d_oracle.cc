@@ -927,3 +927,64 @@ extern "C" void orc_apply_maps(orc_bunch* b, const orc_slice* s, int nslices)
                      p[i + 5 * st], b->ref_p, b->mass);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Moments for diagnostics with the fixed tree (chunks of 4096, 16 per lane,
+// pairwise lanes): mean (calculate_mean) then centred second moments
+// (particle_reducer<mom2_tag>, calculate_mom2).
+// ---------------------------------------------------------------------------
+namespace {
+template <int K, typename F>
+void tree_generic(int64_t n, const uint8_t* masks, F&& add, double out[K])
+{
+    int64_t nch = (n + kTreeChunk - 1) / kTreeChunk;
+    std::vector<double> P(size_t(nch) * K);
+    std::vector<double> s(size_t(kTreeLanes) * K);
+    auto pairwise = [&]() {
+        for (int w = 1; w < kTreeLanes; w *= 2)
+            for (int t = 0; t < kTreeLanes; t += 2 * w)
+                for (int q = 0; q < K; ++q) s[t * K + q] = s[t * K + q] + s[(t + w) * K + q];
+    };
+    for (int64_t c = 0; c < nch; ++c) {
+        std::fill(s.begin(), s.end(), 0.0);
+        for (int t = 0; t < kTreeLanes; ++t)
+            for (int j = 0; j < kTreeItems; ++j) {
+                int64_t i = c * kTreeChunk + int64_t(j) * kTreeLanes + t;
+                if (i >= n || !masks[i]) continue;
+                add(i, &s[t * K]);
+            }
+        pairwise();
+        for (int q = 0; q < K; ++q) P[c * K + q] = s[q];
+    }
+    std::fill(s.begin(), s.end(), 0.0);
+    for (int t = 0; t < kTreeLanes; ++t)
+        for (int64_t c = t; c < nch; c += kTreeLanes)
+            for (int q = 0; q < K; ++q) s[t * K + q] += P[c * K + q];
+    pairwise();
+    for (int q = 0; q < K; ++q) out[q] = s[q];
+}
+}  // namespace
+
+extern "C" void orc_moments(const double* p, int64_t st, int64_t n, const uint8_t* masks,
+                            double total_num, double mean[6], double mom2[36])
+{
+    double s1[6];
+    tree_generic<6>(n, masks, [&](int64_t i, double* acc) {
+        for (int j = 0; j < 6; ++j) acc[j] += p[i + j * st];
+    }, s1);
+    for (int j = 0; j < 6; ++j) mean[j] = s1[j] / total_num;
+    double s2[21];
+    tree_generic<21>(n, masks, [&](int64_t i, double* acc) {
+        int q = 0;
+        for (int j = 0; j < 6; ++j) {
+            double dj = p[i + j * st] - mean[j];
+            for (int k = 0; k <= j; ++k) acc[q++] += dj * (p[i + k * st] - mean[k]);
+        }
+    }, s2);
+    int q = 0;
+    for (int j = 0; j < 6; ++j)
+        for (int k = 0; k <= j; ++k) {
+            mom2[j * 6 + k] = mom2[k * 6 + j] = s2[q] / total_num;
+            ++q;
+        }
+}

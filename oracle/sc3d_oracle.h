@@ -120,6 +120,10 @@ typedef struct orc_slice {
     double length, k1, k1s, ref_cdt;
 } orc_slice;
 double orc_reference_cdt(const orc_slice* s, double ref_p, double mass);
+/* Core_diagnostics::calculate_mean / calculate_mom2 (core_diagnostics.cc:
+ * 202-229,318-362) with the fixed tree of orc_tree_sums. */
+void orc_moments(const double* parts, int64_t stride, int64_t n, const uint8_t* masks,
+                 double total_num, double mean[6], double mom2[36]);
 void orc_apply_maps(orc_bunch* b, const orc_slice* s, int nslices);
 
 #ifdef __cplusplus

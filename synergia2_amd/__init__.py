@@ -123,7 +123,8 @@ PHASES = ("sc3d_domain", "sc3d_sort", "sc3d_local_rho", "sc3d_green_fn2", "sc3d_
 
 EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
-           "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps")
+           "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
+           "sc3d_moments")
 
 _lib = None
 
@@ -154,11 +155,14 @@ def load(path=None):
     L.sc3d_apply_maps.argtypes = [C.POINTER(BunchDesc), C.POINTER(MapSlice), C.c_int, C.c_void_p]
     L.sc3d_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.sc3d_set_post_kick_maps.argtypes = [C.c_void_p, C.POINTER(MapSlice), C.c_int]
+    L.sc3d_moments.argtypes = [C.c_void_p, C.POINTER(BunchDesc), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double), C.c_void_p]
     L.sc3d_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.c_int]
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
                  "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
-                 "sc3d_get_timing", "sc3d_set_post_kick_maps"):
+                 "sc3d_get_timing", "sc3d_set_post_kick_maps",
+           "sc3d_moments"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L
@@ -305,6 +309,15 @@ class SpaceCharge3dOpenHockney:
         arr, n = slices
         _check(load().sc3d_set_post_kick_maps(self._ctx, arr, n))
 
+    def moments(self, bunch, stream=None):
+        """(mean[6], mom2[6,6]) of the bunch, computed on the device."""
+        mean = np.zeros(6)
+        mom2 = np.zeros(36)
+        d = bunch.desc()
+        _check(load().sc3d_moments(self._ctx, C.byref(d), mean.ctypes.data_as(C.POINTER(C.c_double)),
+                                   mom2.ctypes.data_as(C.POINTER(C.c_double)), _stream_ptr(stream)))
+        return mean, mom2.reshape(6, 6)
+
     def set_timing(self, enable=True):
         _check(load().sc3d_set_timing(self._ctx, int(enable)))
 
@@ -352,3 +365,16 @@ def apply_maps(bunch, slices, stream=None):
         return
     d = bunch.desc()
     _check(load().sc3d_apply_maps(C.byref(d), arr, n, _stream_ptr(stream)))
+
+
+def emittances(mom2):
+    """rms emittances (x, y, z) = sqrt(det) of the 2x2 blocks of the second
+    moments (calculate_emittances, diagnostics_full2_host.cc:20-40), with
+    small negatives (> -1e-15) clamped to 0 as the reference does."""
+    out = []
+    for k in (0, 2, 4):
+        det = mom2[k, k] * mom2[k + 1, k + 1] - mom2[k + 1, k] * mom2[k, k + 1]
+        if -1e-15 < det < 0:
+            det = 0.0
+        out.append(float(np.sqrt(det)))
+    return out

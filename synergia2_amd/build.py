@@ -64,9 +64,23 @@ def build_host():
     return HOST_LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "operator_driver.cc")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "operator_driver")
+
+
+def build_driver():
+    """The C++ test driver of the operator class (used by a GPU test)."""
+    _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", HOST, "-I",
+          os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", DRIVER_SRC, "-o", DRIVER,
+          "-L", HERE, "-lsc3d_host", "-lsc3d_hip", "-L", "/opt/rocm/lib", "-lamdhip64",
+          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,/opt/rocm/lib"])
+    return DRIVER
+
+
 def build_all():
     build_hip()
     build_host()
+    build_driver()
 
 
 if __name__ == "__main__":

@@ -68,6 +68,8 @@ struct sc3d_ctx {
     double ph_ms[SC3D_NPHASES] = {0};
     int64_t ph_applies = 0;
     MapProgram post{};  // maps fused into the kick kernel (post.n == 0: none)
+    double* M = nullptr;  // moment partials (diagnostics)
+    int64_t M_cap = 0;
 };
 
 namespace {
@@ -176,7 +178,7 @@ void sc3d_options_default(sc3d_options* o, int32_t gx, int32_t gy, int32_t gz)
     o->domain_fixed = 0;
     o->comm_group_size = 4;
     o->precision = SC3D_PRECISION_FP64;
-    o->sort_period = 1;
+    o->sort_period = 8;
 }
 
 int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
@@ -485,6 +487,38 @@ int sc3d_get_timing(sc3d_ctx* c, double ms[SC3D_NPHASES], int64_t* napplies, int
         for (int p = 0; p < SC3D_NPHASES; ++p) c->ph_ms[p] = 0.0;
         c->ph_applies = 0;
     }
+    return SC3D_OK;
+}
+
+int sc3d_moments(sc3d_ctx* c, const sc3d_bunch* b, double mean[6], double mom2[36],
+                 void* stream)
+{
+    if (!c || !b || !mean || !mom2) return fail(SC3D_ERR_ARG, "sc3d_moments: null argument");
+    const int64_t n = b->n_active;
+    if (n < 0 || b->stride < n || (n > 0 && (!b->parts || !b->masks)))
+        return fail(SC3D_ERR_ARG, "sc3d_moments: bad bunch");
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t need = ((n + 4095) / 4096) * 21 + 64;
+    if (need > c->M_cap) {
+        dfree(c, c->M);
+        if (int rc = dmalloc(c, &c->M, need + 27)) return rc;
+        c->M_cap = need;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double* out = c->M + c->M_cap;  // 27 result doubles after the partials
+    launch_moments(b->parts, b->stride, b->masks, n, b->total_num, c->M, out, st);
+    HIP_TRY(hipGetLastError());
+    double h[27];
+    HIP_TRY(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int j = 0; j < 6; ++j) mean[j] = h[j];
+    int q = 6;
+    for (int j = 0; j < 6; ++j)
+        for (int k = 0; k <= j; ++k) {
+            mom2[j * 6 + k] = h[q];
+            mom2[k * 6 + j] = h[q];
+            ++q;
+        }
     return SC3D_OK;
 }
 

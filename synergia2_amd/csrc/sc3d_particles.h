@@ -61,6 +61,8 @@ template <typename R>
 void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
                  int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
                  double m, const MapProgram& prog, hipStream_t st);
+void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                    double total_num, double* P, double* out, hipStream_t st);
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                      const PopulateArgs& a, hipStream_t st);
 

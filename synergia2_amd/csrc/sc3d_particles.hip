@@ -31,6 +31,104 @@ __device__ __forceinline__ void block_pairwise6(double (&s)[6], double (*ws)[6])
         for (int q = 0; q < 6; ++q) s[q] = (ws[0][q] + ws[1][q]) + (ws[2][q] + ws[3][q]);
 }
 
+template <int K>
+__device__ __forceinline__ void block_pairwiseK(double (&s)[K], double (*ws)[K])
+{
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2)
+#pragma unroll
+        for (int q = 0; q < K; ++q) s[q] = s[q] + __shfl_xor(s[q], w, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) ws[wave][q] = s[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) s[q] = (ws[0][q] + ws[1][q]) + (ws[2][q] + ws[3][q]);
+}
+
+// Moments for diagnostics (Core_diagnostics::calculate_mean / calculate_mom2,
+// core_diagnostics.cc:126-198,202-229,318-362): MODE 0 sums the six
+// coordinates; MODE 1 sums the 21 centred products (p_j - mean_j)(p_k -
+// mean_k), k <= j, with the mean read from `mean`.  Same fixed tree as the
+// bunch extent (chunks of 4096, 16 per lane, pairwise lanes).
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_moments_partial(const double* __restrict__ parts,
+                                                              int64_t stride,
+                                                              const uint8_t* __restrict__ masks,
+                                                              int64_t n,
+                                                              const double* __restrict__ mean,
+                                                              double* __restrict__ P)
+{
+    constexpr int K = MODE == 0 ? 6 : 21;
+    __shared__ double ws[4][K];
+    double s[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) s[q] = 0.0;
+    double m[6] = {0, 0, 0, 0, 0, 0};
+    if (MODE == 1)
+        for (int j = 0; j < 6; ++j) m[j] = mean[j];
+    const int64_t c = blockIdx.x;
+    for (int jj = 0; jj < 16; ++jj) {
+        int64_t i = c * 4096 + int64_t(jj) * 256 + threadIdx.x;
+        if (i < n && masks[i]) {
+            double v[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) v[j] = parts[i + j * stride];
+            if (MODE == 0) {
+#pragma unroll
+                for (int j = 0; j < 6; ++j) s[j] += v[j];
+            } else {
+                int q = 0;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    const double dj = v[j] - m[j];
+#pragma unroll
+                    for (int k = 0; k <= j; ++k) s[q++] += dj * (v[k] - m[k]);
+                }
+            }
+        }
+    }
+    block_pairwiseK<K>(s, ws);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < K; ++q) P[c * K + q] = s[q];
+}
+
+// Combines the chunk partials by the same rule and divides by total_num.
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_moments_finish(const double* __restrict__ P,
+                                                             int64_t nch, double total_num,
+                                                             double* __restrict__ out)
+{
+    __shared__ double ws[4][K];
+    double s[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) s[q] = 0.0;
+    for (int64_t c = threadIdx.x; c < nch; c += kThreads)
+#pragma unroll
+        for (int q = 0; q < K; ++q) s[q] += P[c * K + q];
+    block_pairwiseK<K>(s, ws);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < K; ++q) out[q] = s[q] / total_num;
+}
+
+void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                    double total_num, double* P, double* out /* 6 + 21 */, hipStream_t st)
+{
+    int64_t nch = (n + 4095) / 4096;
+    if (nch > 0)
+        hipLaunchKernelGGL(k_moments_partial<0>, dim3(unsigned(nch)), dim3(kThreads), 0, st,
+                           parts, stride, masks, n, (const double*)nullptr, P);
+    hipLaunchKernelGGL(k_moments_finish<6>, dim3(1), dim3(kThreads), 0, st, P, nch, total_num,
+                       out);
+    if (nch > 0)
+        hipLaunchKernelGGL(k_moments_partial<1>, dim3(unsigned(nch)), dim3(kThreads), 0, st,
+                           parts, stride, masks, n, (const double*)out, P);
+    hipLaunchKernelGGL(k_moments_finish<21>, dim3(1), dim3(kThreads), 0, st, P, nch, total_num,
+                       out + 6);
+}
+
 __global__ __launch_bounds__(kThreads) void k_domain_partial(const double* __restrict__ parts,
                                                              int64_t stride,
                                                              const uint8_t* __restrict__ masks,
