@@ -110,8 +110,8 @@ int ensure_particle_buffers(sc3d_ctx* c, int64_t n)
     rc |= dmalloc(c, &c->sb.keysB, cap);
     rc |= dmalloc(c, &c->sb.valsA, cap);
     rc |= dmalloc(c, &c->sb.valsB, cap);
-    rc |= dmalloc(c, &c->sb.counts, 256 * tiles);
-    rc |= dmalloc(c, &c->sb.totals, 256);
+    rc |= dmalloc(c, &c->sb.counts, 2048 * tiles);
+    rc |= dmalloc(c, &c->sb.totals, 2048);
     rc |= dmalloc(c, &c->scratch, 7 * cap);
     rc |= dmalloc(c, &c->smask, cap);
     if (rc) return rc;

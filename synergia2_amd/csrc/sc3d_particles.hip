@@ -249,51 +249,72 @@ __global__ __launch_bounds__(kThreads) void k_domain_finish(const double* __rest
 // Cell keys: k = (iz*gy + iy)*gx + ix for active particles with lower corner
 // in [0, g-2]^3 (the particles the deposit and the kick touch), else G.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_keys(const double* __restrict__ parts,
-                                                   int64_t stride,
-                                                   const uint8_t* __restrict__ masks,
-                                                   int64_t n, int gx, int gy, int gz,
-                                                   const DomainRec* __restrict__ dom,
-                                                   int32_t* __restrict__ keys)
+__device__ __forceinline__ int32_t cell_key(const double* __restrict__ parts, int64_t stride,
+                                            const uint8_t* __restrict__ masks, int64_t i, int gx,
+                                            int gy, int gz, const DomainRec& d)
 {
-    if (dom->error) return;
-    const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
-    const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
-    const int32_t G = gx * gy * gz;
-    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * kThreads) {
-        double o;
-        int ix = cell_index(parts[i], lx, ihx, o);
-        int iy = cell_index(parts[i + 2 * stride], ly, ihy, o);
-        int iz = cell_index(parts[i + 4 * stride], lz, ihz, o);
-        bool in = masks[i] && ix >= 0 && ix < gx - 1 && iy >= 0 && iy < gy - 1 && iz >= 0 &&
-                  iz < gz - 1;
-        keys[i] = in ? (iz * gy + iy) * gx + ix : G;
-    }
+    double o;
+    const int ix = cell_index(parts[i], d.left[0], d.inv_h[0], o);
+    const int iy = cell_index(parts[i + 2 * stride], d.left[1], d.inv_h[1], o);
+    const int iz = cell_index(parts[i + 4 * stride], d.left[2], d.inv_h[2], o);
+    const bool in = masks[i] && ix >= 0 && ix < gx - 1 && iy >= 0 && iy < gy - 1 && iz >= 0 &&
+                    iz < gz - 1;
+    return in ? (iz * gy + iy) * gx + ix : gx * gy * gz;
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort of (key, index), 8-bit digits, tiles of 4096.
+// Stable LSD radix sort of (key, index): digits of `width` <= 8 bits, tiles
+// of 4096 items (256 lanes x 16 rounds).  In the scatter, equal digits inside
+// a wavefront are matched with `width` ballots, so ranks need one LDS update
+// per distinct digit per wavefront.
 // ---------------------------------------------------------------------------
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kThreads * kSortItems;
+constexpr int kMaxDigitBits = 8;
+constexpr int kMaxBuckets = 1 << kMaxDigitBits;
 
-__global__ __launch_bounds__(kThreads) void k_radix_count(const int32_t* __restrict__ keys,
-                                                          int64_t n, int shift,
-                                                          int32_t* __restrict__ counts,
-                                                          int nblk, const DomainRec* dom)
+// lanes of this wavefront holding the same digit (valid lanes only)
+__device__ __forceinline__ uint64_t match_digit(int d, bool valid, int width)
 {
-    __shared__ int32_t hist[256];
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < width; ++b) {
+        const uint64_t m = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    return peers;
+}
+
+// Histogram of one digit per tile (LDS atomics); KEYS: compute the cell keys
+// from the particles first (and store them), fusing the key pass into pass 0.
+template <bool KEYS>
+__global__ __launch_bounds__(kThreads) void k_radix_count(
+    const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks, int gx,
+    int gy, int gz, int32_t* __restrict__ keys, int64_t n, int shift, int width,
+    int32_t* __restrict__ counts, int nblk, const DomainRec* __restrict__ dom)
+{
+    __shared__ int32_t hist[kMaxBuckets];
     if (dom->error) return;
-    hist[threadIdx.x] = 0;
+    const int nb = 1 << width;
+    for (int e = threadIdx.x; e < nb; e += kThreads) hist[e] = 0;
     __syncthreads();
+    const DomainRec d = *dom;
     const int64_t base = int64_t(blockIdx.x) * kSortTile;
-    for (int j = 0; j < kSortItems; ++j) {
-        int64_t i = base + int64_t(j) * kThreads + threadIdx.x;
-        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 255], 1);
+    for (int r = 0; r < kSortItems; ++r) {
+        const int64_t i = base + int64_t(r) * kThreads + threadIdx.x;
+        const bool valid = i < n;
+        int32_t key = 0;
+        if (valid) {
+            if (KEYS) {
+                key = cell_key(parts, stride, masks, i, gx, gy, gz, d);
+                keys[i] = key;
+            } else {
+                key = keys[i];
+            }
+        }
+        if (valid) atomicAdd(&hist[(key >> shift) & (nb - 1)], 1);
     }
     __syncthreads();
-    counts[int64_t(threadIdx.x) * nblk + blockIdx.x] = hist[threadIdx.x];
+    for (int e = threadIdx.x; e < nb; e += kThreads) counts[int64_t(e) * nblk + blockIdx.x] = hist[e];
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts (in
@@ -329,66 +350,76 @@ __global__ __launch_bounds__(kThreads) void k_radix_scan(int32_t* __restrict__ c
 }
 
 // Stable scatter.  Items are ranked in tile order: 16 rounds of 256; inside
-// a round, wavefront w holds items [64w, 64w+64); the rank of an item among
-// equal digits of its wavefront comes from 8 ballots.
+// a round, wavefront w holds items [64w, 64w+64); an item's rank among equal
+// digits of its wavefront comes from the ballot match; the lowest lane of
+// each digit group publishes the group size and later advances the running
+// count and clears its slot (no full clears per round).
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int64_t n,
-    int shift, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals, int nblk,
-    int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, const DomainRec* dom)
+    int shift, int width, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals,
+    int nblk, int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
+    const DomainRec* dom)
 {
-    __shared__ int32_t base_d[256];
-    __shared__ int32_t run_d[256];
-    __shared__ int32_t wave_d[4][256];
+    __shared__ int32_t base_d[kMaxBuckets];
+    __shared__ int32_t run_d[kMaxBuckets];
+    __shared__ int32_t wave_d[4][kMaxBuckets];
+    __shared__ int32_t chunk_sum[kThreads];
     if (dom->error) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    // digit base = exclusive prefix of totals + this tile's scanned count
-    {
-        int32_t v = totals[t];
-        base_d[t] = v;
-        __syncthreads();
-        for (int w = 1; w < 256; w *= 2) {
-            int32_t u = t >= w ? base_d[t - w] : 0;
-            __syncthreads();
-            base_d[t] += u;
-            __syncthreads();
-        }
-        int32_t excl = base_d[t] - v + counts[int64_t(t) * nblk + blockIdx.x];
-        __syncthreads();
-        base_d[t] = excl;
-        run_d[t] = 0;
+    const int nb = 1 << width;
+    const int per = nb / kThreads > 0 ? nb / kThreads : 1;  // nb is 256..2048 or smaller
+    // exclusive prefix of the digit totals (block scan over nb entries)
+    int32_t local = 0;
+    for (int k = 0; k < per; ++k) {
+        const int e = t * per + k;
+        if (e < nb) local += totals[e];
     }
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    chunk_sum[t] = local;
+    __syncthreads();
+    for (int w = 1; w < kThreads; w *= 2) {
+        int32_t v = t >= w ? chunk_sum[t - w] : 0;
+        __syncthreads();
+        chunk_sum[t] += v;
+        __syncthreads();
+    }
+    int32_t run = chunk_sum[t] - local;
+    for (int k = 0; k < per; ++k) {
+        const int e = t * per + k;
+        if (e < nb) {
+            base_d[e] = run + counts[int64_t(e) * nblk + blockIdx.x];
+            run += totals[e];
+            run_d[e] = 0;
+            wave_d[0][e] = wave_d[1][e] = wave_d[2][e] = wave_d[3][e] = 0;
+        }
+    }
+    __syncthreads();
+    const uint64_t lt_mask = (1ull << lane) - 1;
     const int64_t tile = int64_t(blockIdx.x) * kSortTile;
     for (int r = 0; r < kSortItems; ++r) {
-        wave_d[0][t] = 0;
-        wave_d[1][t] = 0;
-        wave_d[2][t] = 0;
-        wave_d[3][t] = 0;
-        __syncthreads();
-        int64_t i = tile + int64_t(r) * kThreads + t;
-        bool valid = i < n;
-        int32_t key = valid ? keys_in[i] : 0;
-        int32_t val = valid ? (vals_in ? vals_in[i] : int32_t(i)) : 0;
-        int d = (key >> shift) & 255;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            uint64_t m = __ballot((d >> b) & 1);
-            peers &= ((d >> b) & 1) ? m : ~m;
-        }
-        int rank = __popcll(peers & lt_mask);
-        if (valid && rank == 0) wave_d[wave][d] = __popcll(peers);
+        const int64_t i = tile + int64_t(r) * kThreads + t;
+        const bool valid = i < n;
+        const int32_t key = valid ? keys_in[i] : 0;
+        const int32_t val = valid ? (vals_in ? vals_in[i] : int32_t(i)) : 0;
+        const int dg = (key >> shift) & (nb - 1);
+        const uint64_t peers = match_digit(dg, valid, width);
+        const int rank = __popcll(peers & lt_mask);
+        const bool leader = valid && rank == 0;
+        if (leader) wave_d[wave][dg] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            int64_t pos = base_d[d] + run_d[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wave_d[w][d];
+            int64_t pos = base_d[dg] + run_d[dg] + rank;
+            for (int w = 0; w < wave; ++w) pos += wave_d[w][dg];
             if (pos >= 0 && pos < n) {  // always true for consistent counts
                 keys_out[pos] = key;
                 vals_out[pos] = val;
             }
         }
         __syncthreads();
-        run_d[t] += wave_d[0][t] + wave_d[1][t] + wave_d[2][t] + wave_d[3][t];
+        if (leader) {
+            atomicAdd(&run_d[dg], __popcll(peers));
+            wave_d[wave][dg] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -578,7 +609,9 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
     for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kThreads) {
         if (!masks[i]) continue;
+        // all six coordinates are loaded up front (independent loads in flight)
         double x = parts[i], y = parts[i + 2 * stride], cdt = parts[i + 4 * stride];
+        double xp = parts[i + stride], yp = parts[i + 3 * stride], dpop = parts[i + 5 * stride];
         double ox, oy, oz;
         const int ix = cell_index(x, lx, ihx, ox);
         const int iy = cell_index(y, ly, ihy, oy);
@@ -586,7 +619,6 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
         const bool in = (ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) &&
                         (iz >= 0 && iz < gz - 1);
         if (!MAPS && !in) continue;
-        double xp = parts[i + stride], yp = parts[i + 3 * stride], dpop = parts[i + 5 * stride];
         if (in) {
             const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
             const int64_t b0 = (int64_t(iz) * gy + iy) * gx + ix;
@@ -704,28 +736,43 @@ int sort_passes(int64_t G)
 {
     int bits = 0;
     while ((int64_t(1) << bits) <= G) ++bits;  // keys in [0, G]
-    return (bits + 7) / 8;
+    return (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+}
+
+static int sort_bits(int64_t G)
+{
+    int bits = 0;
+    while ((int64_t(1) << bits) <= G) ++bits;
+    return bits;
 }
 
 void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
                  int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride, masks, n,
-                       gx, gy, gz, dom, b.keys0);
     const int nblk = int((n + kSortTile - 1) / kSortTile);
-    const int passes = sort_passes(int64_t(gx) * gy * gz);
+    const int bits = sort_bits(int64_t(gx) * gy * gz);
+    const int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const int width = (bits + passes - 1) / passes;
+    const int nb = 1 << width;
     const int32_t* kin = b.keys0;
     const int32_t* vin = nullptr;
     int32_t* kout[2] = {b.keysA, b.keysB};
     int32_t* vout[2] = {b.valsA, b.valsB};
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_count, dim3(nblk), dim3(kThreads), 0, st, kin, n, shift,
-                           b.counts, nblk, dom);
-        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(kThreads), 0, st, b.counts, nblk,
+        const int shift = width * p;
+        if (p == 0)
+            hipLaunchKernelGGL(k_radix_count<true>, dim3(nblk), dim3(kThreads), 0, st, parts,
+                               stride, masks, gx, gy, gz, b.keys0, n, shift, width, b.counts,
+                               nblk, dom);
+        else
+            hipLaunchKernelGGL(k_radix_count<false>, dim3(nblk), dim3(kThreads), 0, st,
+                               (const double*)nullptr, int64_t(0), (const uint8_t*)nullptr, gx,
+                               gy, gz, const_cast<int32_t*>(kin), n, shift, width, b.counts,
+                               nblk, dom);
+        hipLaunchKernelGGL(k_radix_scan, dim3(nb), dim3(kThreads), 0, st, b.counts, nblk,
                            b.totals, dom);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kThreads), 0, st, kin, vin, n, shift,
-                           b.counts, b.totals, nblk, kout[p & 1], vout[p & 1], dom);
+                           width, b.counts, b.totals, nblk, kout[p & 1], vout[p & 1], dom);
         kin = kout[p & 1];
         vin = vout[p & 1];
     }
