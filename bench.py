@@ -61,7 +61,11 @@ def main():
     ap.add_argument("--config", default="fodo_fp64_4M_64x64x128", choices=sorted(CONFIGS))
     ap.add_argument("--sort-period", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: time space-charge steps until this much CPU time")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first time a 1 GiB device-to-device copy (counter calibration and "
+                         "achievable HBM bandwidth)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -74,6 +78,8 @@ def main():
 
     import synergia2_amd as sc
     from synergia2_amd import stepper as S
+
+    copy_gbps = copy_probe(torch) if args.copy_probe else None
 
     n, grid, pop, seed, sigz, sigdp, emit, precision = CONFIGS[args.config]
     ke = 0.8
@@ -121,26 +127,31 @@ def main():
         print(f"#   {k:16s} {v / args.steps * 1e3:9.2f} us/step", file=sys.stderr)
 
     w = 8 if precision == 0 else 4
-    cands = [(phases[p], p) for p in ("sc3d_kick", "sc3d_local_rho")]
-    dom_ms, dom_phase = max(cands)
-    per_launch_s = dom_ms / args.steps / 1e3
-    B = algorithmic_bytes(dom_phase, n, grid, w)
-    achieved = B / per_launch_s / 1e9
-    roofline = {
-        "bound": "hbm",
-        "kernel": {"sc3d_kick": "k_kick", "sc3d_local_rho": "k_deposit"}[dom_phase],
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBPS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBPS, 4),
-        "traffic": None,
-        "algorithmic_bytes_per_launch": B,
-        "launch_us": round(per_launch_s * 1e6, 2),
-    }
+    traffic = measured_traffic(args.config)
+    lines = []
+    for phase, kname in (("sc3d_kick", "k_kick"), ("sc3d_local_rho", "k_deposit")):
+        per_launch_s = phases[phase] / args.steps / 1e3
+        B = algorithmic_bytes(phase, n, grid, w)
+        achieved = B / per_launch_s / 1e9
+        lines.append({
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic.get(kname),
+            "algorithmic_bytes_per_launch": B,
+            "launch_us": round(per_launch_s * 1e6, 2),
+        })
+    # the dominant kernel: the longer of the two particle passes
+    roofline = dict(max(lines, key=lambda r: r["launch_us"]))
+    roofline["copy_probe_GBps"] = copy_gbps
+    roofline["particle_kernels"] = lines
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bunch, prop, opts, args.cpu_steps, sc, S)
+        cpu = cpu_baseline(bunch, prop, opts, args.cpu_seconds, sc, S)
 
     out = {
         "metric": "particle-kicks/s (3D Hockney SC step)",
@@ -169,9 +180,53 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(bunch, prop, opts, steps, sc, S):
-    """The oracle (CPU restatement of the reference operator + maps) on the
-    host cores, same bunch and steps, FP64.  Bounded to a few steps."""
+def measured_traffic(config):
+    """HBM bytes per launch of k_kick / k_deposit from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this workload (calibrated
+    against a copy of known size), or {} when none was taken for it."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_sc_step_{config}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        doc = json.load(f)
+    out = {}
+    for name, rec in doc["kernels"].items():
+        if "k_kick<double, false>" in name or "k_kick<float, false>" in name:
+            out["k_kick"] = rec["traffic_bytes"]
+        elif "k_deposit" in name:
+            out["k_deposit"] = rec["traffic_bytes"]
+    return out
+
+
+def copy_probe(torch):
+    """A 1 GiB device-to-device copy (512 MiB read + 512 MiB written), timed
+    with events over 5 repeats: the achievable HBM rate of a plain stream,
+    and the kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate those of
+    the space-charge kernels."""
+    nbytes = 512 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbps = 5 * 2 * nbytes / (e0.elapsed_time(e1) / 1e3) / 1e9
+    print(f"# copy probe: {2 * nbytes >> 20} MiB per copy, {gbps:.1f} GB/s", file=sys.stderr,
+          flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbps, 1)
+
+
+def cpu_baseline(bunch, prop, opts, seconds, sc, S):
+    """The oracle (CPU restatement of the reference operator) on the host
+    cores, continuing the same bunch through the same channel in FP64: the
+    maps between kicks run untimed, the space-charge applies are timed, until
+    they have taken `seconds` (at least 2 steps)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -183,26 +238,29 @@ def cpu_baseline(bunch, prop, opts, steps, sc, S):
                      bunch.gamma, bunch.beta, bunch.p)
     o = O.make_options(tuple(opts.shape), n_sigma=opts.n_sigma)
     k0 = prop.i
-    t0 = time.perf_counter()
-    for j in range(steps):
-        k = (k0 + j) % len(prop.steps)
+    sc_time = 0.0
+    steps = 0
+    while steps < 2 or (sc_time < seconds and steps < 400):
+        k = (k0 + steps) % len(prop.steps)
         first, second, length = prop.steps[k]
         O.apply_maps(b, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, bunch.p, bunch.mass))
                          for kd, L, k1 in first])
+        t0 = time.perf_counter()
         rc, _ = O.apply(o, b, length / (bunch.beta * sc.C_LIGHT), nthreads=cores)
+        sc_time += time.perf_counter() - t0
         assert rc == 0, O.last_error()
         O.apply_maps(b, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, bunch.p, bunch.mass))
                          for kd, L, k1 in second])
-    dt = time.perf_counter() - t0
+        steps += 1
     return {
-        "value": bunch.n * steps / dt,
+        "value": bunch.n * steps / sc_time,
         "unit": "particle-kicks/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{steps} split-operator steps of the same workload ({bunch.n} particles, "
-                  f"grid {tuple(opts.shape)}), oracle/ CPU restatement of the reference "
-                  f"operator with OpenMP ({cores} threads) and its own FFT (FFTW absent); "
-                  f"{dt:.2f} s",
+        "sample": f"{steps} space-charge steps of the same workload ({bunch.n} particles, "
+                  f"grid {tuple(opts.shape)}, FODO channel; maps between kicks untimed), "
+                  f"oracle/ CPU restatement of the reference operator with OpenMP ({cores} "
+                  f"threads) and its own FFT (FFTW absent); {sc_time:.2f} s of space-charge time",
     }
 
 
