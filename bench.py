@@ -34,10 +34,42 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 
 CONFIGS = {
-    # name: (n, grid, population, seed, sigma_z [m], sigma_dpop, emit, precision)
-    "fodo_fp64_4M_64x64x128": (4_000_000, (64, 64, 128), 1e11, 1, 0.010, 1e-4, 1e-6, 0),
-    "fodo_fp32grid_16M_128x128x256": (16_000_000, (128, 128, 256), 2e11, 2, 0.020, 1e-4, 1e-6, 1),
+    # FP64 form: 4e6 particles matched to the cell (rms emittance 1 mm mrad),
+    # sigma_z 10 mm, population 1e11, seed 1, grid 64x64x128
+    "fodo_fp64_4M_64x64x128": dict(n=4_000_000, grid=(64, 64, 128), pop=1e11, seed=1,
+                                   sigma_z=0.010, sigma_dpop=1e-4, emit=1e-6, precision=0),
+    # precision form: 1.6e7 particles, sigma_x = sigma_y = 2 mm at the cell
+    # entrance (matched), sigma_z 20 mm, population 2e11, seed 2, grid
+    # 128x128x256 with FP32 deposit / FFT / field
+    "fodo_fp32grid_16M_128x128x256": dict(n=16_000_000, grid=(128, 128, 256), pop=2e11, seed=2,
+                                          sigma_z=0.020, sigma_dpop=1e-4, sigma_xy=0.002,
+                                          precision=1),
 }
+KINETIC_ENERGY = 0.8  # GeV, proton
+
+
+def make_workload(sc, S, config, precision=None, sort_period=None):
+    """Bunch + operator + propagator of a named workload (FODO channel of 4
+    cells, 10 kicks per cell); precision overrides the config's form."""
+    import numpy as np
+    cfg = CONFIGS[config]
+    gamma = (sc.PROTON_MASS + KINETIC_ENERGY) / sc.PROTON_MASS
+    beta, _ = sc.kinematics(sc.PROTON_MASS, gamma)
+    cell = S.fodo_channel(1)
+    if "sigma_xy" in cfg:
+        chol = S.matched_chol_sigma(cell, cfg["sigma_xy"], cfg["sigma_xy"],
+                                    cfg["sigma_z"] / beta, cfg["sigma_dpop"])
+    else:
+        chol = S.matched_chol(cell, cfg["emit"], cfg["sigma_z"] / beta, cfg["sigma_dpop"])
+    bunch = sc.Bunch(cfg["n"], cfg["pop"], gamma=gamma)
+    bunch.populate_gaussian(np.zeros(6), chol, seed=cfg["seed"])
+    opts = sc.default_options(*cfg["grid"])
+    opts.precision = cfg["precision"] if precision is None else precision
+    if sort_period is not None:
+        opts.sort_period = sort_period
+    op = sc.SpaceCharge3dOpenHockney(opts)
+    prop = S.Propagator(S.fodo_channel(4), 40, bunch, op)
+    return bunch, opts, op, prop
 
 
 def algorithmic_bytes(phase, n, g, w):
@@ -63,6 +95,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline: time space-charge steps until this much CPU time")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="also state field rel-L2 and rms-emittance drift of the FP32-grid form "
+                         "against the FP64 form and the oracle on this workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a 1 GiB device-to-device copy (counter calibration and "
                          "achievable HBM bandwidth)")
@@ -81,21 +116,9 @@ def main():
 
     copy_gbps = copy_probe(torch) if args.copy_probe else None
 
-    n, grid, pop, seed, sigz, sigdp, emit, precision = CONFIGS[args.config]
-    ke = 0.8
-    gamma = (sc.PROTON_MASS + ke) / sc.PROTON_MASS
-    beta, _ = sc.kinematics(sc.PROTON_MASS, gamma)
-    lattice = S.fodo_channel(4)
-    chol = S.matched_chol(S.fodo_channel(1), emit, sigz / beta, sigdp)
-
-    bunch = sc.Bunch(n, pop, gamma=gamma)
-    bunch.populate_gaussian(np.zeros(6), chol, seed=seed)
-    opts = sc.default_options(*grid)
-    opts.precision = precision
-    if args.sort_period is not None:
-        opts.sort_period = args.sort_period
-    op = sc.SpaceCharge3dOpenHockney(opts)
-    prop = S.Propagator(lattice, 40, bunch, op)
+    cfg = CONFIGS[args.config]
+    n, grid, precision = cfg["n"], cfg["grid"], cfg["precision"]
+    bunch, opts, op, prop = make_workload(sc, S, args.config, sort_period=args.sort_period)
 
     print(f"# {args.config}: n={n} grid={grid} warmup={args.warmup} steps={args.steps} "
           f"sort_period={opts.sort_period}", file=sys.stderr, flush=True)
@@ -153,6 +176,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(bunch, prop, opts, args.cpu_seconds, sc, S)
 
+    acc = None
+    if args.accuracy:
+        del op, prop, bunch
+        torch.cuda.empty_cache()
+        acc = accuracy_report(sc, S, args.config, args.steps)
+
     out = {
         "metric": "particle-kicks/s (3D Hockney SC step)",
         "value": value,
@@ -177,7 +206,66 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if acc is not None:
+        out["accuracy"] = acc
     print(json.dumps(out), flush=True)
+
+
+def rel_l2(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def accuracy_report(sc, S, config, steps):
+    """The FP32-grid form against the FP64 form on the same seeded bunch:
+    field rel-L2 (Ex, Ey, Ez octants) at the first kick and relative rms
+    emittance drift (x, y, z) after `steps` steps; and the first kick's field
+    of both forms against the oracle (CPU restatement, FP64) on the bunch
+    state right before that kick."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    taps = (sc.DBG_EX, sc.DBG_EY, sc.DBG_EZ)
+    field, emit = {}, {}
+    for prec in (0, 1):
+        bunch, opts, op, prop = make_workload(sc, S, config, precision=prec)
+        if prec == 0:
+            # oracle: first-step maps then one space-charge apply on the host
+            parts, masks = bunch.host()
+            parts = np.ascontiguousarray(parts)
+            masks = np.ascontiguousarray(masks)
+            ob = O.make_bunch(parts, masks, bunch.total_num, bunch.real_num, bunch.charge,
+                              bunch.mass, bunch.gamma, bunch.beta, bunch.p)
+            first, _, length = prop.steps[0]
+            O.apply_maps(ob, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, bunch.p, bunch.mass))
+                              for kd, L, k1 in first])
+            t0 = time.perf_counter()
+            rc, out = O.apply(O.make_options(tuple(opts.shape), n_sigma=opts.n_sigma), ob,
+                              length / (bunch.beta * sc.C_LIGHT), nthreads=min(os.cpu_count(), 16),
+                              want=("ex", "ey", "ez"))
+            assert rc == 0, O.last_error()
+            oracle_field = [out["ex"], out["ey"], out["ez"]]
+            print(f"# oracle apply {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            del parts, masks, ob, out
+        e0 = sc.emittances(op.moments(bunch)[1])
+        prop.run(1, sync=True, fuse=False)
+        field[prec] = [op.debug(t) for t in taps]
+        prop.run(steps - 1, sync=True)
+        emit[prec] = (e0, sc.emittances(op.moments(bunch)[1]))
+        del bunch, op, prop
+        torch.cuda.empty_cache()
+    rep = {
+        "steps": steps,
+        "field_rel_l2_fp32_vs_fp64": max(rel_l2(field[1][c], field[0][c]) for c in range(3)),
+        "field_rel_l2_fp64_vs_oracle": max(rel_l2(field[0][c], oracle_field[c]) for c in range(3)),
+        "field_rel_l2_fp32_vs_oracle": max(rel_l2(field[1][c], oracle_field[c]) for c in range(3)),
+        "emittance_rel_drift_fp32_vs_fp64": [abs(emit[1][1][k] / emit[0][1][k] - 1)
+                                             for k in range(3)],
+        "emittance_growth_fp64": [emit[0][1][k] / emit[0][0][k] for k in range(3)],
+    }
+    print(f"# accuracy: {json.dumps(rep)}", file=sys.stderr, flush=True)
+    return rep
 
 
 def measured_traffic(config):

@@ -121,6 +121,8 @@ const char* sc3d_last_error(void);
 #define SC3D_DBG_PERM 2          /* int32[n] stable sort permutation         */
 #define SC3D_DBG_RHO 3           /* double[G] charge density octant (zyx)    */
 #define SC3D_DBG_PHI 4           /* double[G] potential octant, unnormalised */
+                                 /* (FP64 form; the precision form forms no  */
+                                 /* potential and returns SC3D_ERR_ARG)      */
 #define SC3D_DBG_EX 5            /* double[G]                                */
 #define SC3D_DBG_EY 6
 #define SC3D_DBG_EZ 7
@@ -161,7 +163,9 @@ typedef struct sc3d_domain_record {
     double w0, fn_norm, factor;
     double inv_h[3], half_inv_h[3];
     double ghat_h[3];            /* cell size the cached spectrum is for     */
-    int32_t error, ghat_dirty, ghat_valid, pad;
+    int32_t error, ghat_dirty, ghat_valid;
+    int32_t rho_peak;            /* precision form: bits of the FP32 density */
+                                 /* peak (unit charge weights), 0 otherwise  */
 } sc3d_domain_record;
 
 /* Single-particle maps of a FODO channel, applied on the device as one

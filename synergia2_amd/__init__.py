@@ -99,7 +99,7 @@ class DomainRecord(C.Structure):
         ("error", C.c_int32),
         ("ghat_dirty", C.c_int32),
         ("ghat_valid", C.c_int32),
-        ("pad", C.c_int32),
+        ("rho_peak", C.c_int32),
     ]
 
     def as_dict(self):

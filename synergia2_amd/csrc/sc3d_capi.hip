@@ -3,7 +3,7 @@
 // spectrum of one operator instance, and the per-apply launch sequence of
 // Space_charge_3d_open_hockney::apply_bunch
 // (space_charge_3d_open_hockney.cc:422-453):
-//   domain -> [cell sort + reorder] -> deposit -> Green's spectrum (only when
+//   domain -> cell sort [+ reorder] -> deposit -> Green's spectrum (only when
 //   h changed) -> x/y/z FFT passes with the convolution fused into the z pass
 //   -> E field -> kick.
 // Everything is enqueued on the caller's stream; no host synchronisation
@@ -13,87 +13,16 @@
 #include <string>
 #include <vector>
 
-#include "sc3d_particles.h"
+#include "sc3d_ctx.h"
 
 using namespace sc3d;
 
 namespace {
 thread_local std::string g_err;
 
-int fail(int code, const std::string& msg)
-{
-    g_err = msg;
-    return code;
-}
-
 bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
 
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(SC3D_ERR_HIP, std::string(#expr " failed: ") +                 \
-                                          hipGetErrorString(e_));                      \
-    } while (0)
-}  // namespace
-
-struct sc3d_ctx {
-    sc3d_options opt;
-    Geometry g;
-    int device = 0;
-    bool fp32 = false;
-    DomainRec* dom = nullptr;
-    double* P = nullptr;
-    int64_t P_cap = 0;
-    void* rho = nullptr;
-    void* S = nullptr;
-    void* phi = nullptr;
-    void* E = nullptr;
-    double* ghat64 = nullptr;
-    float* ghat32 = nullptr;
-    double2* tw64 = nullptr;
-    float2* tw32 = nullptr;
-    int64_t pcap = 0;
-    SortBuffers sb{};
-    double* scratch = nullptr;
-    uint8_t* smask = nullptr;
-    int64_t applies = 0;
-    int64_t sorted_n = -1;
-    hipStream_t last_stream = nullptr;
-    std::vector<void*> owned;
-    // phase timing: one set of SC3D_NPHASES+1 events per timed apply
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    double ph_ms[SC3D_NPHASES] = {0};
-    int64_t ph_applies = 0;
-    MapProgram post{};  // maps fused into the kick kernel (post.n == 0: none)
-    double* M = nullptr;  // moment partials (diagnostics)
-    int64_t M_cap = 0;
-};
-
-namespace {
-
-template <typename T>
-int dmalloc(sc3d_ctx* c, T** p, int64_t n)
-{
-    void* q = nullptr;
-    if (n <= 0) n = 1;
-    HIP_TRY(hipMalloc(&q, size_t(n) * sizeof(T)));
-    c->owned.push_back(q);
-    *p = static_cast<T*>(q);
-    return 0;
-}
-
-void dfree(sc3d_ctx* c, void* p)
-{
-    if (!p) return;
-    for (auto& q : c->owned)
-        if (q == p) {
-            (void)hipFree(q);
-            q = nullptr;
-        }
-}
+#define HIP_TRY SC3D_HIP_TRY
 
 int ensure_particle_buffers(sc3d_ctx* c, int64_t n)
 {
@@ -138,7 +67,38 @@ void make_twiddles(std::vector<double2>& t64, std::vector<float2>& t32)
         }
 }
 
-// Records the next pooled event on st (timing on) and returns it.
+Geometry geometry_of(const sc3d_options& o)
+{
+    Geometry g;
+    g.gx = o.shape[0]; g.gy = o.shape[1]; g.gz = o.shape[2];
+    g.Dx = 2 * g.gx; g.Dy = 2 * g.gy; g.Dz = 2 * g.gz;
+    g.Hx = g.gx + 1;
+    g.G = int64_t(g.gx) * g.gy * g.gz;
+    g.S_elems = int64_t(g.gz) * g.Dy * g.Hx;
+    g.Ghat_elems = int64_t(g.gz + 1) * (g.gy + 1) * g.Hx;
+    return g;
+}
+
+}  // namespace
+
+namespace sc3d {
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+void dfree(sc3d_ctx* c, void* p)
+{
+    if (!p) return;
+    for (auto& q : c->owned)
+        if (q == p) {
+            (void)hipFree(q);
+            q = nullptr;
+        }
+}
+
 hipEvent_t mark(sc3d_ctx* c, hipStream_t st)
 {
     if (!c->timing) return nullptr;
@@ -152,7 +112,156 @@ hipEvent_t mark(sc3d_ctx* c, hipStream_t st)
     return e;
 }
 
-}  // namespace
+int64_t ctx_footprint(const sc3d_options& o, int64_t n)
+{
+    const Geometry g = geometry_of(o);
+    const bool fp32 = o.precision == SC3D_PRECISION_FP32_GRID;
+    const int64_t rs = fp32 ? 4 : 8;
+    int64_t b = sizeof(DomainRec) + sizeof(GridJob) + sizeof(StepCtl);
+    b += g.G * rs;                               // rho
+    b += g.S_elems * 2 * rs * (fp32 ? 2 : 1);    // S (+ S2)
+    b += fp32 ? 0 : g.G * rs;                    // phi
+    b += g.G * 4 * rs;                           // E
+    b += g.Ghat_elems * (8 + (fp32 ? 4 : 0));    // Green's spectrum (+ FP32 copy)
+    b += kTwiddleElems * (16 + 8);               // twiddles
+    const int64_t cap = n + n / 8 + 4096;        // ensure_particle_buffers
+    const int64_t tiles = (cap + kSortTileItems - 1) / kSortTileItems;
+    b += cap * 4 * 5 + 2048 * tiles * 4 + 2048 * 4 + cap * 8 * 7 + cap;
+    b += (((n + 4095) / 4096) * 6 + 64) * 8;     // domain partials
+    return b;
+}
+
+int prepare_apply(sc3d_ctx* c, const sc3d_bunch* b)
+{
+    if (!c || !b) return fail(SC3D_ERR_ARG, "sc3d_apply: null argument");
+    const int64_t n = b->n_active;
+    if (n < 0 || b->stride < n) return fail(SC3D_ERR_ARG, "sc3d_apply: bad n_active/stride");
+    if (n > 0 && (!b->parts || !b->masks))
+        return fail(SC3D_ERR_ARG, "sc3d_apply: null particle or mask pointer");
+    if (n >= (int64_t(1) << 31) - kSortTileItems)
+        return fail(SC3D_ERR_SIZE, "sc3d_apply: more than 2^31 particles in one bunch");
+    if (c->opt.periodic_z)
+        return fail(SC3D_ERR_PERIODIC,
+                    c->opt.green_fn == SC3D_GREEN_POINTLIKE
+                        ? "Space_charge_3d_open_hockney::get_green_fn2_pointlike: periodic_z "
+                          "not yet implemented"
+                        : "Space_charge_3d_open_hockney::get_green_fn2_linear: periodic_z not "
+                          "yet implemented");
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t nch = (n + 4095) / 4096;
+    if (!c->opt.domain_fixed && nch * 6 > c->P_cap) {
+        dfree(c, c->P);
+        if (int rc = dmalloc(c, &c->P, nch * 6 + 64)) return rc;
+        c->P_cap = nch * 6 + 64;
+    }
+    if (n > 0)
+        if (int rc = ensure_particle_buffers(c, n)) return rc;
+    return SC3D_OK;
+}
+
+int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sched& sc,
+                  hipStream_t st, hipEvent_t* ev)
+{
+    const int64_t n = b->n_active;
+    const Geometry& g = c->g;
+    const sc3d_options& o = c->opt;
+    DomainFinishArgs a{};
+    a.fixed = o.domain_fixed;
+    a.green_fn = o.green_fn;
+    a.grid_entire_period = o.grid_entire_period;
+    a.charge = b->charge;
+    for (int k = 0; k < 3; ++k) {
+        a.shape[k] = o.shape[k];
+        a.size[k] = o.size[k];
+        a.offset[k] = o.offset[k];
+    }
+    a.n_sigma = o.n_sigma;
+    a.z_period = o.z_period;
+    a.kick_scale = o.kick_scale;
+    a.total_num = b->total_num;
+    a.real_num = b->real_num;
+    a.time_step = time_step;
+    a.ref_p = b->ref_p;
+    a.ref_gamma = b->ref_gamma;
+    a.ref_beta = b->ref_beta;
+    // the cell sort runs on every apply (the deposit walks the particles in
+    // cell order); the bunch storage is reordered every sort_period applies
+    // (decided here, or on the device under step control)
+    const bool do_sort = n > 0;
+    const bool do_reorder =
+        do_sort && o.sort_period > 0 &&
+        (sc.ctl != nullptr || (c->applies % o.sort_period) == 0);
+    ev[0] = mark(c, st);
+    launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, sc, st);
+    ev[1] = mark(c, st);
+    const int32_t* dep_perm = nullptr;
+    if (do_sort) {
+        launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb, st);
+        c->sorted_n = n;
+        if (do_reorder)
+            launch_reorder(b->parts, b->stride, b->masks, n, c->sb.perm, c->scratch, c->smask,
+                           c->dom, sc.ctl, st);
+        // under step control the deposit itself skips the permutation on a
+        // reorder step
+        if (!do_reorder || sc.ctl) dep_perm = c->sb.perm;
+    }
+    if (!c->fp32) {
+        launch_fill<double>(static_cast<double*>(c->rho), g.G, st);
+        ev[2] = mark(c, st);
+        launch_deposit<double>(b->parts, b->stride, b->masks, dep_perm, n, g.gx, g.gy, g.gz,
+                               c->dom, static_cast<double*>(c->rho), sc.ctl, st);
+    } else {
+        launch_fill<float>(static_cast<float*>(c->rho), g.G, st);
+        ev[2] = mark(c, st);
+        launch_deposit<float>(b->parts, b->stride, b->masks, dep_perm, n, g.gx, g.gy, g.gz,
+                              c->dom, static_cast<float*>(c->rho), sc.ctl, st);
+    }
+    ev[3] = mark(c, st);
+    return SC3D_OK;
+}
+
+void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream_t st,
+                  hipEvent_t* ev)
+{
+    sc3d_ctx* c = ctxs[0];
+    const Geometry& g = c->g;
+    if (!c->fp32) {
+        launch_green<double>(g, c->opt.green_fn, table, nj, c->tw64, st);
+        ev[4] = mark(c, st);
+        launch_solve<double>(g, table, nj, c->tw64, st);
+        ev[5] = mark(c, st);
+        for (int j = 0; j < nj; ++j) {
+            sc3d_ctx* cj = ctxs[j];
+            launch_efield<double>(static_cast<double*>(cj->phi), g.gx, g.gy, g.gz, cj->dom,
+                                  static_cast<double4*>(cj->E), st);
+        }
+        ev[6] = mark(c, st);
+    } else {
+        launch_green<float>(g, c->opt.green_fn, table, nj, c->tw64, st);
+        ev[4] = mark(c, st);
+        // the field comes straight out of the spectrum (no potential, no
+        // separate difference pass): sc3d_force stays empty in this form
+        launch_solve_field<float>(g, table, nj, c->tw32, st);
+        ev[5] = mark(c, st);
+        ev[6] = mark(c, st);
+    }
+}
+
+void enqueue_kick(sc3d_ctx* c, const sc3d_bunch* b, const Sched& sc, hipStream_t st)
+{
+    const Geometry& g = c->g;
+    MapProgram post = c->post;
+    post.ref_p = b->ref_p;
+    post.mass = b->mass;
+    if (!c->fp32)
+        launch_kick<double>(b->parts, b->stride, b->masks, b->n_active, g.gx, g.gy, g.gz, c->dom,
+                            static_cast<double4*>(c->E), b->ref_p, b->mass, post, sc, st);
+    else
+        launch_kick<float>(b->parts, b->stride, b->masks, b->n_active, g.gx, g.gy, g.gz, c->dom,
+                           static_cast<float4*>(c->E), b->ref_p, b->mass, post, sc, st);
+}
+
+}  // namespace sc3d
 
 extern "C" {
 
@@ -198,13 +307,8 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     sc3d_ctx* c = new sc3d_ctx();
     c->opt = *opt;
     c->fp32 = opt->precision == SC3D_PRECISION_FP32_GRID;
-    Geometry& g = c->g;
-    g.gx = gx; g.gy = gy; g.gz = gz;
-    g.Dx = 2 * gx; g.Dy = 2 * gy; g.Dz = 2 * gz;
-    g.Hx = gx + 1;
-    g.G = int64_t(gx) * gy * gz;
-    g.S_elems = int64_t(gz) * g.Dy * g.Hx;
-    g.Ghat_elems = int64_t(gz + 1) * (gy + 1) * g.Hx;
+    c->g = geometry_of(*opt);
+    const Geometry& g = c->g;
     auto cleanup = [&](int rc) {
         sc3d_destroy(c);
         return rc;
@@ -219,15 +323,29 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     c->rho = p;
     rc |= dmalloc(c, &p, g.S_elems * 2 * rs);
     c->S = p;
-    rc |= dmalloc(c, &p, g.G * rs);
-    c->phi = p;
+    if (c->fp32) {
+        rc |= dmalloc(c, &p, g.S_elems * 2 * rs);
+        c->S2 = p;
+    } else {
+        rc |= dmalloc(c, &p, g.G * rs);
+        c->phi = p;
+    }
     rc |= dmalloc(c, &p, g.G * 4 * rs);
     c->E = p;
     rc |= dmalloc(c, &c->ghat64, g.Ghat_elems);
     if (c->fp32) rc |= dmalloc(c, &c->ghat32, g.Ghat_elems);
     rc |= dmalloc(c, &c->tw64, kTwiddleElems);
     rc |= dmalloc(c, &c->tw32, kTwiddleElems);
+    rc |= dmalloc(c, &c->d_gj, 1);
     if (rc) return cleanup(rc);
+    c->gj.rho = c->rho;
+    c->gj.S = c->S;
+    c->gj.S2 = c->S2;
+    c->gj.phi = c->phi;
+    c->gj.E = c->E;
+    c->gj.ghat64 = c->ghat64;
+    c->gj.ghat = c->fp32 ? static_cast<void*>(c->ghat32) : static_cast<void*>(c->ghat64);
+    c->gj.dom = c->dom;
     std::vector<double2> t64;
     std::vector<float2> t32;
     make_twiddles(t64, t32);
@@ -235,6 +353,7 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
             hipSuccess ||
         hipMemcpy(c->tw32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice) !=
             hipSuccess ||
+        hipMemcpy(c->d_gj, &c->gj, sizeof(GridJob), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->dom, 0, sizeof(DomainRec)) != hipSuccess)
         return cleanup(fail(SC3D_ERR_HIP, "sc3d_create: workspace initialisation failed"));
     *out = c;
@@ -254,114 +373,13 @@ int sc3d_destroy(sc3d_ctx* c)
 
 int sc3d_apply(sc3d_ctx* c, const sc3d_bunch* b, double time_step, void* stream)
 {
-    if (!c || !b) return fail(SC3D_ERR_ARG, "sc3d_apply: null argument");
-    const int64_t n = b->n_active;
-    if (n < 0 || b->stride < n) return fail(SC3D_ERR_ARG, "sc3d_apply: bad n_active/stride");
-    if (n > 0 && (!b->parts || !b->masks))
-        return fail(SC3D_ERR_ARG, "sc3d_apply: null particle or mask pointer");
-    if (n >= (int64_t(1) << 31) - kSortTileItems)
-        return fail(SC3D_ERR_SIZE, "sc3d_apply: more than 2^31 particles in one bunch");
-    if (c->opt.periodic_z)
-        return fail(SC3D_ERR_PERIODIC,
-                    c->opt.green_fn == SC3D_GREEN_POINTLIKE
-                        ? "Space_charge_3d_open_hockney::get_green_fn2_pointlike: periodic_z "
-                          "not yet implemented"
-                        : "Space_charge_3d_open_hockney::get_green_fn2_linear: periodic_z not "
-                          "yet implemented");
-    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = prepare_apply(c, b)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const Geometry& g = c->g;
-    const sc3d_options& o = c->opt;
-
-    // bunch extent -> device domain record
-    const int64_t nch = (n + 4095) / 4096;
-    if (!o.domain_fixed && nch * 6 > c->P_cap) {
-        dfree(c, c->P);
-        if (int rc = dmalloc(c, &c->P, nch * 6 + 64)) return rc;
-        c->P_cap = nch * 6 + 64;
-    }
-    DomainFinishArgs a{};
-    a.fixed = o.domain_fixed;
-    a.green_fn = o.green_fn;
-    a.grid_entire_period = o.grid_entire_period;
-    a.charge = b->charge;
-    for (int k = 0; k < 3; ++k) {
-        a.shape[k] = o.shape[k];
-        a.size[k] = o.size[k];
-        a.offset[k] = o.offset[k];
-    }
-    a.n_sigma = o.n_sigma;
-    a.z_period = o.z_period;
-    a.kick_scale = o.kick_scale;
-    a.total_num = b->total_num;
-    a.real_num = b->real_num;
-    a.time_step = time_step;
-    a.ref_p = b->ref_p;
-    a.ref_gamma = b->ref_gamma;
-    a.ref_beta = b->ref_beta;
-    // the cell sort runs on every apply (the deposit walks the particles in
-    // cell order); the bunch storage is reordered every sort_period applies
-    const bool do_sort = n > 0;
-    const bool do_reorder =
-        do_sort && o.sort_period > 0 && (c->applies % o.sort_period) == 0;
-    if (do_sort)
-        if (int rc = ensure_particle_buffers(c, n)) return rc;
-    MapProgram post = c->post;
-    post.ref_p = b->ref_p;
-    post.mass = b->mass;
+    const Sched none{};
     hipEvent_t ev[SC3D_NPHASES + 1];
-    ev[0] = mark(c, st);
-    launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, st);
-    ev[1] = mark(c, st);
-
-    // cell sort + in-place reorder of the bunch
-    const int32_t* dep_perm = nullptr;
-    if (do_sort) {
-        launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb, st);
-        c->sorted_n = n;
-        if (do_reorder)
-            launch_reorder(b->parts, b->stride, b->masks, n, c->sb.perm, c->scratch, c->smask,
-                           c->dom, st);
-        else
-            dep_perm = c->sb.perm;
-    }
-    if (!c->fp32)
-        launch_fill<double>(static_cast<double*>(c->rho), g.G, st);
-    else
-        launch_fill<float>(static_cast<float*>(c->rho), g.G, st);
-    ev[2] = mark(c, st);
-
-    if (!c->fp32) {
-        double* rho = static_cast<double*>(c->rho);
-        launch_deposit<double>(b->parts, b->stride, b->masks, dep_perm, n, g.gx, g.gy, g.gz,
-                               c->dom, rho, st);
-        ev[3] = mark(c, st);
-        launch_green<double>(g, o.green_fn, c->dom, c->tw64, c->ghat64, c->ghat64, st);
-        ev[4] = mark(c, st);
-        launch_solve<double>(g, rho, static_cast<double2*>(c->S), c->ghat64, c->tw64,
-                             static_cast<double*>(c->phi), c->dom, st);
-        ev[5] = mark(c, st);
-        launch_efield<double>(static_cast<double*>(c->phi), g.gx, g.gy, g.gz, c->dom,
-                              static_cast<double4*>(c->E), st);
-        ev[6] = mark(c, st);
-        launch_kick<double>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom,
-                            static_cast<double4*>(c->E), b->ref_p, b->mass, post, st);
-    } else {
-        float* rho = static_cast<float*>(c->rho);
-        launch_deposit<float>(b->parts, b->stride, b->masks, dep_perm, n, g.gx, g.gy, g.gz,
-                              c->dom, rho, st);
-        ev[3] = mark(c, st);
-        launch_green<float>(g, o.green_fn, c->dom, c->tw64, c->ghat64, c->ghat32, st);
-        ev[4] = mark(c, st);
-        launch_solve<float>(g, rho, static_cast<float2*>(c->S), c->ghat32, c->tw32,
-                            static_cast<float*>(c->phi), c->dom, st);
-        ev[5] = mark(c, st);
-        launch_efield<float>(static_cast<float*>(c->phi), g.gx, g.gy, g.gz, c->dom,
-                             static_cast<float4*>(c->E), st);
-        ev[6] = mark(c, st);
-        launch_kick<float>(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom,
-                           static_cast<float4*>(c->E), b->ref_p, b->mass, post, st);
-    }
+    enqueue_front(c, b, time_step, none, st, ev);
+    enqueue_grid(&c, 1, c->d_gj, st, ev);
+    enqueue_kick(c, b, none, st);
     ev[7] = mark(c, st);
     (void)ev;
     HIP_TRY(hipGetLastError());
@@ -392,15 +410,28 @@ int sc3d_debug_get(sc3d_ctx* c, int what, void* dst, int64_t bytes)
     HIP_TRY(hipStreamSynchronize(c->last_stream));
     const Geometry& g = c->g;
     auto need = [&](int64_t nb) -> bool { return bytes >= nb; };
+    // FP32 grids are returned in the FP64 form's units: the density grid
+    // holds unit weights (x w0), the field comes from the density scaled to
+    // unit peak (x w0 * peak)
+    auto fp32_scale = [&](bool peak_scaled, double& scale) -> int {
+        DomainRec r;
+        HIP_TRY(hipMemcpy(&r, c->dom, sizeof(r), hipMemcpyDeviceToHost));
+        float peak;
+        std::memcpy(&peak, &r.rho_peak, sizeof(peak));
+        scale = peak_scaled && peak > 0.0f ? r.w0 * double(peak) : r.w0;
+        return SC3D_OK;
+    };
     auto grid_out = [&](const void* src) -> int {
         if (!need(g.G * 8)) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: buffer too small");
         if (!c->fp32) {
             HIP_TRY(hipMemcpy(dst, src, g.G * 8, hipMemcpyDeviceToHost));
         } else {
+            double scale = 1.0;
+            if (int rc = fp32_scale(false, scale)) return rc;
             std::vector<float> tmp(g.G);
             HIP_TRY(hipMemcpy(tmp.data(), src, g.G * 4, hipMemcpyDeviceToHost));
             double* d = static_cast<double*>(dst);
-            for (int64_t i = 0; i < g.G; ++i) d[i] = tmp[i];
+            for (int64_t i = 0; i < g.G; ++i) d[i] = double(tmp[i]) * scale;
         }
         return SC3D_OK;
     };
@@ -418,7 +449,12 @@ int sc3d_debug_get(sc3d_ctx* c, int what, void* dst, int64_t bytes)
         return SC3D_OK;
     }
     case SC3D_DBG_RHO: return grid_out(c->rho);
-    case SC3D_DBG_PHI: return grid_out(c->phi);
+    case SC3D_DBG_PHI:
+        if (c->fp32)
+            return fail(SC3D_ERR_ARG,
+                        "sc3d_debug_get: the precision form transforms the field straight "
+                        "from the spectrum and keeps no potential");
+        return grid_out(c->phi);
     case SC3D_DBG_EX:
     case SC3D_DBG_EY:
     case SC3D_DBG_EZ: {
@@ -430,9 +466,11 @@ int sc3d_debug_get(sc3d_ctx* c, int what, void* dst, int64_t bytes)
             HIP_TRY(hipMemcpy(tmp.data(), c->E, g.G * 32, hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < g.G; ++i) d[i] = tmp[4 * i + comp];
         } else {
+            double scale = 1.0;
+            if (int rc = fp32_scale(true, scale)) return rc;
             std::vector<float> tmp(g.G * 4);
             HIP_TRY(hipMemcpy(tmp.data(), c->E, g.G * 16, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < g.G; ++i) d[i] = tmp[4 * i + comp];
+            for (int64_t i = 0; i < g.G; ++i) d[i] = double(tmp[4 * i + comp]) * scale;
         }
         return SC3D_OK;
     }

@@ -30,7 +30,8 @@ struct DomainRec {
     double w0, fn_norm, factor;
     double inv_h[3], half_inv_h[3];
     double ghat_h[3];
-    int32_t error, ghat_dirty, ghat_valid, pad;
+    int32_t error, ghat_dirty, ghat_valid;
+    int32_t rho_peak;  // precision form: FP32 bits of the unit-weight density peak
 };
 static_assert(sizeof(DomainRec) == sizeof(sc3d_domain_record), "layout");
 
@@ -50,6 +51,30 @@ __device__ __forceinline__ int cell_index(double pos, double left, double inv_h,
     off = s - idx;
     return idx;
 }
+
+// Device-side control of the applies of one context in the served form
+// (kernels take it as a nullable pointer; null = the host decides, as in
+// sc3d_apply).  k_step_begin opens each apply: whether it reorders the bunch
+// storage (every sort_period applies from the first), which scheduled map
+// programs run before / after the kick and which kick time step applies,
+// and whether the job has finished (then every kernel of the apply exits).
+// The launch sequence of an apply is then the same every step, so one
+// captured graph serves every step.
+struct StepCtl {
+    int64_t applies;      // applies opened so far
+    int32_t sort_period;
+    int32_t reorder;      // this apply reorders the bunch storage
+    int32_t turn_steps;   // schedule length (steps per turn)
+    int32_t total_steps;  // steps of the job
+    int32_t cur;          // schedule index of this apply
+    int32_t first_prog;   // program table index before this kick, -1: none
+    int32_t after_prog;   // program table index fused after this kick
+    int32_t done;         // all steps taken: this apply does nothing
+};
+
+// Domain-record error value of an apply of a finished job (served form):
+// every kernel of that apply exits.  Not a C-ABI error code.
+constexpr int32_t kDoneMark = 1;
 
 template <typename R> struct cplx;
 template <> struct cplx<double> { using type = double2; };

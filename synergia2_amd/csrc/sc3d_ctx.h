@@ -1,0 +1,102 @@
+// sc3d_ctx.h -- the operator context (device workspaces of one operator
+// instance, Space_charge_3d_open_hockney's rho2/g2/phi2/En* and FFT plans,
+// space_charge_3d_open_hockney.h:22-37) and the three pieces one apply is
+// made of, shared by the C ABI (sc3d_capi.hip) and the served form
+// (sc3d_serve.hip).  Internal to libsc3d_hip.so.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "sc3d_particles.h"
+
+struct sc3d_ctx {
+    sc3d_options opt;
+    sc3d::Geometry g;
+    int device = 0;
+    bool fp32 = false;
+    sc3d::DomainRec* dom = nullptr;
+    double* P = nullptr;
+    int64_t P_cap = 0;
+    void* rho = nullptr;
+    void* S = nullptr;
+    void* S2 = nullptr;   // precision form: second spectrum workspace
+    void* phi = nullptr;  // FP64 form only
+    void* E = nullptr;
+    double* ghat64 = nullptr;
+    float* ghat32 = nullptr;
+    double2* tw64 = nullptr;
+    float2* tw32 = nullptr;
+    sc3d::GridJob gj{};               // this context's grid pointers
+    sc3d::GridJob* d_gj = nullptr;    // the same, as a device table of one
+    int64_t pcap = 0;
+    sc3d::SortBuffers sb{};
+    double* scratch = nullptr;
+    uint8_t* smask = nullptr;
+    int64_t applies = 0;
+    int64_t sorted_n = -1;
+    hipStream_t last_stream = nullptr;
+    std::vector<void*> owned;
+    // phase timing: one set of SC3D_NPHASES+1 events per timed apply
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double ph_ms[SC3D_NPHASES] = {0};
+    int64_t ph_applies = 0;
+    sc3d::MapProgram post{};  // maps fused into the kick kernel (post.n == 0: none)
+    double* M = nullptr;      // moment partials (diagnostics)
+    int64_t M_cap = 0;
+};
+
+namespace sc3d {
+
+// Sets this thread's last error message (sc3d_last_error) and returns code.
+int fail(int code, const std::string& msg);
+
+#define SC3D_HIP_TRY(expr)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::sc3d::fail(SC3D_ERR_HIP, std::string(#expr " failed: ") +            \
+                                                  hipGetErrorString(e_));                  \
+    } while (0)
+
+template <typename T>
+int dmalloc(sc3d_ctx* c, T** p, int64_t n)
+{
+    void* q = nullptr;
+    if (n <= 0) n = 1;
+    SC3D_HIP_TRY(hipMalloc(&q, size_t(n) * sizeof(T)));
+    c->owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+void dfree(sc3d_ctx* c, void* p);
+
+// Records the next pooled timing event on st (timing on), else nullptr.
+hipEvent_t mark(sc3d_ctx* c, hipStream_t st);
+
+// Device bytes a context of these options holds once it has applied to a
+// bunch of n particles (the admission footprint of the served form).
+int64_t ctx_footprint(const sc3d_options& o, int64_t n);
+
+// Argument checks of an apply and the particle-side buffers for its bunch.
+int prepare_apply(sc3d_ctx* c, const sc3d_bunch* b);
+
+// The particle front of an apply: domain, cell sort, reorder, density
+// (events ev[0..3] when timing).  With sc.ctl (served form) the step
+// control decides reorder, schedule and kick time step on the device.
+int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sched& sc,
+                  hipStream_t st, hipEvent_t* ev);
+
+// The grid middle for nj contexts of one shape and precision: Green's
+// spectrum (where h changed), convolution and field, as batched launches
+// over the device table `table` (events ev[4..6] when timing).
+void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream_t st,
+                  hipEvent_t* ev);
+
+// The kick (with the fused post-kick maps: c->post, or the schedule's).
+void enqueue_kick(sc3d_ctx* c, const sc3d_bunch* b, const Sched& sc, hipStream_t st);
+
+}  // namespace sc3d

@@ -137,18 +137,25 @@ __device__ __forceinline__ void lds_fft(T2* buf, const T2* __restrict__ tw)
 // is non-zero only for n < M/2.
 // ---------------------------------------------------------------------------
 template <typename R, int M>
-__global__ __launch_bounds__(kThreads) void k_x_r2c(const R* __restrict__ rho,
-                                                    typename cplx<R>::type* __restrict__ S,
+__global__ __launch_bounds__(kThreads) void k_x_r2c(const GridJob* __restrict__ jobs,
                                                     const typename cplx<R>::type* __restrict__ twM,
                                                     const typename cplx<R>::type* __restrict__ tw2M,
-                                                    int nrows, int gy, int Dy, const DomainRec* dom)
+                                                    int nrows, int gy, int Dy)
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = Lines<M>::value;
     __shared__ T2 buf[NL * (M + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    const R* __restrict__ rho = static_cast<const R*>(J.rho);
+    T2* __restrict__ S = static_cast<T2*>(J.S);
+    const DomainRec* dom = J.dom;
     if (dom->error) return;
     const int r0 = blockIdx.x * NL;
     const int Hx = M + 1;
+    // precision form: the unit-weight density is scaled to unit peak here,
+    // before any transform arithmetic (the peak returns at the kick)
+    const float peak = __int_as_float(dom->rho_peak);
+    const bool unit_peak = sizeof(R) == sizeof(float) && peak > 0.0f;
     // load: NL consecutive rows are NL*M contiguous reals = NL*M/2 complex
     const T2* src = reinterpret_cast<const T2*>(rho + int64_t(r0) * M);
     for (int e = threadIdx.x; e < NL * M; e += kThreads) {
@@ -156,6 +163,10 @@ __global__ __launch_bounds__(kThreads) void k_x_r2c(const R* __restrict__ rho,
         T2 v;
         if (n < M / 2 && r0 + line < nrows) {
             v = src[line * (M / 2) + n];
+            if (unit_peak) {
+                v.x = R(v.x / peak);
+                v.y = R(v.y / peak);
+            }
         } else {
             v.x = R(0); v.y = R(0);
         }
@@ -190,15 +201,18 @@ __global__ __launch_bounds__(kThreads) void k_x_r2c(const R* __restrict__ rho,
 // z = inverse FFT_M(Z'); x[2n] = Re z[n], x[2n+1] = Im z[n]; keep n < M/2.
 // ---------------------------------------------------------------------------
 template <typename R, int M>
-__global__ __launch_bounds__(kThreads) void k_x_c2r(const typename cplx<R>::type* __restrict__ S,
-                                                    R* __restrict__ phi,
+__global__ __launch_bounds__(kThreads) void k_x_c2r(const GridJob* __restrict__ jobs,
                                                     const typename cplx<R>::type* __restrict__ twM,
                                                     const typename cplx<R>::type* __restrict__ tw2M,
-                                                    int nrows, int gy, int Dy, const DomainRec* dom)
+                                                    int nrows, int gy, int Dy)
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = Lines<M>::value;
     __shared__ T2 buf[NL * (M + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    const T2* __restrict__ S = static_cast<const T2*>(J.S);
+    R* __restrict__ phi = static_cast<R*>(J.phi);
+    const DomainRec* dom = J.dom;
     if (dom->error) return;
     const int r0 = blockIdx.x * NL;
     const int Hx = M + 1;
@@ -239,16 +253,19 @@ __global__ __launch_bounds__(kThreads) void k_x_c2r(const typename cplx<R>::type
 // the real Green's spectrum, inverse (the z pass).
 // ---------------------------------------------------------------------------
 template <typename R, int L, int MODE>
-__global__ __launch_bounds__(kThreads) void k_cols(typename cplx<R>::type* __restrict__ S,
+__global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ jobs,
                                                    const typename cplx<R>::type* __restrict__ tw,
-                                                   const R* __restrict__ ghat, int nlines, int lpp,
-                                                   int64_t plane_stride, int64_t pstride,
-                                                   int nload, int nstore, int gy, int Hx,
-                                                   const DomainRec* dom)
+                                                   int nlines, int lpp, int64_t plane_stride,
+                                                   int64_t pstride, int nload, int nstore, int gy,
+                                                   int Hx)
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = Lines<L>::value;
     __shared__ T2 buf[NL * (L + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    T2* __restrict__ S = static_cast<T2*>(J.S);
+    const R* __restrict__ ghat = static_cast<const R*>(J.ghat);
+    const DomainRec* dom = J.dom;
     if (dom->error) return;
     const int l0 = blockIdx.x * NL;
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
@@ -299,6 +316,223 @@ __global__ __launch_bounds__(kThreads) void k_cols(typename cplx<R>::type* __res
 }
 
 // ---------------------------------------------------------------------------
+// Precision form: the field straight from the spectrum.  The reference's
+// centred difference E_a[i] = -(phi[i+1] - phi[i-1]) * (0.5/h_a) on interior
+// nodes only reaches octant values, so on the doubled (periodic) grid it is
+// exactly the spectral multiplier
+//     D_a(k) = -i sin(2 pi k / D_a) / h_a,     E_a = IDFT(phi^ * D_a).
+// In FP32 the three fields are transformed back from the spectrum instead of
+// differencing a rounded potential: the inverse transforms' rounding is then
+// relative to the field itself and not to the much larger potential, which
+// keeps the field ~30x closer to the FP64 one.  sin(2 pi k / L) is -Im of
+// the twiddle e^{-2 pi i k / L} of the length-L table.
+//   z : per (ky, kx) column: forward, x Ghat, inverse of T and of T*D_z,
+//       keep z < gz: T -> S, T_z -> S2
+//   y': per (z, kx) column: inverse of T, T*D_y and T_z, keep y < gy:
+//       S rows [0, gy) <- T, S rows [gy, Dy) <- T*D_y, S2 rows [0, gy) <- T_z
+//   x': per (z, y) row: C2R of (S row)*D_x, S row gy+y, S2 row -> E (AoS,
+//       interior nodes; boundary nodes 0, as the reference leaves them)
+// ---------------------------------------------------------------------------
+
+// lines per block for K lines held per transform line, within 64 KiB of LDS
+template <int L, int K, typename T2> struct FieldLines {
+    static constexpr int base = Lines<L>::value;
+    static constexpr int value =
+        (size_t(K) * base * (L + 1) * sizeof(T2) <= 65536) ? base : base / 2;
+};
+
+// v * (-i s) = (s v.y, -s v.x)
+template <typename T2, typename R>
+__device__ __forceinline__ T2 mul_minus_i(T2 v, R s)
+{
+    T2 o;
+    o.x = s * v.y;
+    o.y = -(s * v.x);
+    return o;
+}
+
+// sin(2 pi k / L) from the length-L twiddle table, exactly 0 at k = 0, L/2
+template <typename T2>
+__device__ __forceinline__ auto sin_2pi(const T2* __restrict__ tw, int k, int L)
+{
+    return (k == 0 || 2 * k == L) ? decltype(tw[0].y)(0) : -tw[k].y;
+}
+
+template <typename R, int L>
+__global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict__ jobs,
+                                                      const typename cplx<R>::type* __restrict__ tw,
+                                                      int nlines, int gz, int gy, int Hx)
+{
+    using T2 = typename cplx<R>::type;
+    constexpr int NL = FieldLines<L, 2, T2>::value;
+    __shared__ T2 buf[2 * NL * (L + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    T2* __restrict__ S = static_cast<T2*>(J.S);
+    T2* __restrict__ S2 = static_cast<T2*>(J.S2);
+    const R* __restrict__ ghat = static_cast<const R*>(J.ghat);
+    const DomainRec* dom = J.dom;
+    if (dom->error) return;
+    const R ihz = R(dom->inv_h[2]);
+    const int l0 = blockIdx.x * NL;
+    const int Dy = nlines / Hx;
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        T2 v;
+        if (p < gz && l < nlines) {
+            v = S[int64_t(p) * nlines + l];
+        } else {
+            v.x = R(0); v.y = R(0);
+        }
+        buf[line * (L + 1) + p] = v;
+    }
+    __syncthreads();
+    lds_fft<T2, L, NL, false>(buf, tw);
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        if (l >= nlines) continue;
+        int ky = l / Hx, kx = l - ky * Hx;
+        int my = ky <= Dy / 2 ? ky : Dy - ky;
+        int mz = p <= L / 2 ? p : L - p;
+        R gv = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
+        T2 v = buf[line * (L + 1) + p];
+        v.x = v.x * gv;
+        v.y = v.y * gv;
+        buf[line * (L + 1) + p] = v;
+        buf[(NL + line) * (L + 1) + p] = mul_minus_i(v, sin_2pi(tw, p, L) * ihz);
+    }
+    __syncthreads();
+    lds_fft<T2, L, 2 * NL, true>(buf, tw);
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        if (p < gz && l < nlines) {
+            S[int64_t(p) * nlines + l] = buf[line * (L + 1) + p];
+            S2[int64_t(p) * nlines + l] = buf[(NL + line) * (L + 1) + p];
+        }
+    }
+}
+
+template <typename R, int L>
+__global__ __launch_bounds__(kThreads) void k_y_field(const GridJob* __restrict__ jobs,
+                                                      const typename cplx<R>::type* __restrict__ tw,
+                                                      int nlines, int gy, int Hx)
+{
+    using T2 = typename cplx<R>::type;
+    constexpr int NL = FieldLines<L, 3, T2>::value;
+    __shared__ T2 buf[3 * NL * (L + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    T2* __restrict__ S = static_cast<T2*>(J.S);
+    T2* __restrict__ S2 = static_cast<T2*>(J.S2);
+    const DomainRec* dom = J.dom;
+    if (dom->error) return;
+    const R ihy = R(dom->inv_h[1]);
+    const int l0 = blockIdx.x * NL;
+    const int64_t plane = int64_t(L) * Hx;  // L = Dy rows of Hx
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        T2 v, vz;
+        if (l < nlines) {
+            int z = l / Hx, c = l - z * Hx;
+            int64_t at = z * plane + int64_t(p) * Hx + c;
+            v = S[at];
+            vz = S2[at];
+        } else {
+            v.x = R(0); v.y = R(0);
+            vz = v;
+        }
+        buf[line * (L + 1) + p] = v;
+        buf[(NL + line) * (L + 1) + p] = mul_minus_i(v, sin_2pi(tw, p, L) * ihy);
+        buf[(2 * NL + line) * (L + 1) + p] = vz;
+    }
+    __syncthreads();
+    lds_fft<T2, L, 3 * NL, true>(buf, tw);
+    for (int e = threadIdx.x; e < NL * gy; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        if (l < nlines) {
+            int z = l / Hx, c = l - z * Hx;
+            int64_t at = z * plane + int64_t(p) * Hx + c;
+            S[at] = buf[line * (L + 1) + p];
+            S[at + int64_t(gy) * Hx] = buf[(NL + line) * (L + 1) + p];
+            S2[at] = buf[(2 * NL + line) * (L + 1) + p];
+        }
+    }
+}
+
+template <typename R, int M>
+__global__ __launch_bounds__(kThreads) void k_x_field(
+    const GridJob* __restrict__ jobs, const typename cplx<R>::type* __restrict__ twM,
+    const typename cplx<R>::type* __restrict__ tw2M, int gy, int gz, int Dy)
+{
+    using T2 = typename cplx<R>::type;
+    using T4 = typename vec4_of<R>::type;
+    constexpr int NL = FieldLines<M, 3, T2>::value;
+    __shared__ T2 buf[3 * NL * (M + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    const T2* __restrict__ S = static_cast<const T2*>(J.S);
+    const T2* __restrict__ S2 = static_cast<const T2*>(J.S2);
+    T4* __restrict__ E = static_cast<T4*>(J.E);
+    const DomainRec* dom = J.dom;
+    if (dom->error) return;
+    const R ihx = R(dom->inv_h[0]);
+    const int nrows = gz * gy;
+    const int r0 = blockIdx.x * NL;
+    const int Hx = M + 1;
+    for (int e = threadIdx.x; e < NL * M; e += kThreads) {
+        int line = e / M, k = e % M;
+        int r = r0 + line;
+        int z = r / gy, y = r - z * gy;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            T2 v;
+            if (r < nrows) {
+                const T2* row = f == 2 ? S2 + (int64_t(z) * Dy + y) * Hx
+                                       : S + (int64_t(z) * Dy + y + (f == 1 ? gy : 0)) * Hx;
+                T2 xk = row[k];
+                T2 xm = row[M - k];
+                if (k == 0) { xk.y = R(0); xm.y = R(0); }
+                if (f == 0) {  // x derivative, D_x(k) for k and M-k (2M = Dx)
+                    xk = mul_minus_i(xk, sin_2pi(tw2M, k, 2 * M) * ihx);
+                    xm = mul_minus_i(xm, sin_2pi(tw2M, M - k, 2 * M) * ihx);
+                }
+                T2 A, B;
+                A.x = xk.x + xm.x; A.y = xk.y - xm.y;
+                B.x = xk.x - xm.x; B.y = xk.y + xm.y;
+                T2 Cc = cmul_conj(B, tw2M[k]);
+                v.x = A.x - Cc.y;
+                v.y = A.y + Cc.x;
+            } else {
+                v.x = R(0); v.y = R(0);
+            }
+            buf[(f * NL + line) * (M + 1) + k] = v;
+        }
+    }
+    __syncthreads();
+    lds_fft<T2, M, 3 * NL, true>(buf, twM);
+    for (int e = threadIdx.x; e < NL * M; e += kThreads) {
+        int line = e / M, x = e % M;
+        int r = r0 + line;
+        if (r >= nrows) continue;
+        int z = r / gy, y = r - z * gy;
+        T4 o;
+        o.x = R(0); o.y = R(0); o.z = R(0); o.w = R(0);
+        if (x > 0 && x < M - 1 && y > 0 && y < gy - 1 && z > 0 && z < gz - 1) {
+            const int n = x >> 1;
+            T2 a = buf[line * (M + 1) + n];
+            T2 b = buf[(NL + line) * (M + 1) + n];
+            T2 c = buf[(2 * NL + line) * (M + 1) + n];
+            o.x = (x & 1) ? a.y : a.x;
+            o.y = (x & 1) ? b.y : b.x;
+            o.z = (x & 1) ? c.y : c.x;
+        }
+        E[int64_t(r) * M + x] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Green's functions (space_charge_3d_open_hockney.cc:136-348).  Value at the
 // doubled-grid point (X, Y, Z) is f(x, y, z) with x = min(X, Dx-X)*hx etc.,
 // exactly the value the reference's mirror writes leave there.
@@ -338,16 +572,18 @@ __device__ double green_linear(double x, double y, double z, double hz, double G
 // G1: rows (Z<=gz, Y<=gy) of the doubled grid: generate, R2C along x, keep
 // the real part of kx = 0..M (the row is even, its spectrum real).
 template <int M>
-__global__ __launch_bounds__(kThreads) void k_green_x(double* __restrict__ ghat,
+__global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict__ jobs,
                                                       const double2* __restrict__ twM,
                                                       const double2* __restrict__ tw2M,
-                                                      int gy, int gz, int green_fn,
-                                                      const DomainRec* dom)
+                                                      int gy, int gz, int green_fn)
 {
     constexpr int NL = Lines<M>::value;
     constexpr int Dx = 2 * M;
     __shared__ double2 buf[NL * (M + 1)];
     __shared__ double row[NL * (M + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    double* __restrict__ ghat = J.ghat64;
+    const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
     const int nrows = (gz + 1) * (gy + 1);
     const int r0 = blockIdx.x * NL;
@@ -409,13 +645,15 @@ __global__ __launch_bounds__(kThreads) void k_green_x(double* __restrict__ ghat,
 // AXIS 1 (y): address (plane*(gy+1) + p)*Hx + kx, plane = Z <= gz.
 // AXIS 2 (z): address (p*(gy+1) + plane)*Hx + kx, plane = ky <= gy.
 template <int L, int AXIS>
-__global__ __launch_bounds__(kThreads) void k_green_cols(double* __restrict__ ghat,
+__global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restrict__ jobs,
                                                          const double2* __restrict__ tw,
-                                                         int nlines, int gy, int Hx,
-                                                         const DomainRec* dom)
+                                                         int nlines, int gy, int Hx)
 {
     constexpr int NL = Lines<L>::value;
     __shared__ double2 buf[NL * (L + 1)];
+    const GridJob& J = jobs[blockIdx.y];
+    double* __restrict__ ghat = J.ghat64;
+    const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
     const int l0 = blockIdx.x * NL;
     auto addr = [&](int l, int p) -> int64_t {
@@ -441,9 +679,12 @@ __global__ __launch_bounds__(kThreads) void k_green_cols(double* __restrict__ gh
 }
 
 template <typename R>
-__global__ void k_cast_ghat(const double* __restrict__ src, R* __restrict__ dst, int64_t n,
-                            const DomainRec* dom)
+__global__ void k_cast_ghat(const GridJob* __restrict__ jobs, int64_t n)
 {
+    const GridJob& J = jobs[blockIdx.y];
+    const double* __restrict__ src = J.ghat64;
+    R* __restrict__ dst = static_cast<R*>(J.ghat);
+    const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * blockDim.x)
@@ -471,123 +712,164 @@ void dispatch_len(int L, A&&... a)
 inline int nblocks(int64_t items, int per) { return int((items + per - 1) / per); }
 
 template <int M> struct GreenX {
-    static void run(const Geometry& g, int green_fn, const DomainRec* dom, const double2* tw,
-                    double* ghat, hipStream_t st)
+    static void run(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
+                    const double2* tw, hipStream_t st)
     {
         int rows = (g.gz + 1) * (g.gy + 1);
-        hipLaunchKernelGGL(k_green_x<M>, dim3(nblocks(rows, Lines<M>::value)), dim3(kThreads), 0,
-                           st, ghat, tw + twiddle_offset(M), tw + twiddle_offset(2 * M), g.gy,
-                           g.gz, green_fn, dom);
+        hipLaunchKernelGGL(k_green_x<M>, dim3(nblocks(rows, Lines<M>::value), nj),
+                           dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
+                           tw + twiddle_offset(2 * M), g.gy, g.gz, green_fn);
     }
 };
 template <int L> struct GreenY {
-    static void run(const Geometry& g, const DomainRec* dom, const double2* tw, double* ghat,
+    static void run(const Geometry& g, const GridJob* jobs, int nj, const double2* tw,
                     hipStream_t st)
     {
         int lines = (g.gz + 1) * g.Hx;
-        hipLaunchKernelGGL((k_green_cols<L, 1>), dim3(nblocks(lines, Lines<L>::value)),
-                           dim3(kThreads), 0, st, ghat, tw + twiddle_offset(L), lines, g.gy,
-                           g.Hx, dom);
+        hipLaunchKernelGGL((k_green_cols<L, 1>), dim3(nblocks(lines, Lines<L>::value), nj),
+                           dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
+                           g.Hx);
     }
 };
 template <int L> struct GreenZ {
-    static void run(const Geometry& g, const DomainRec* dom, const double2* tw, double* ghat,
+    static void run(const Geometry& g, const GridJob* jobs, int nj, const double2* tw,
                     hipStream_t st)
     {
         int lines = (g.gy + 1) * g.Hx;
-        hipLaunchKernelGGL((k_green_cols<L, 2>), dim3(nblocks(lines, Lines<L>::value)),
-                           dim3(kThreads), 0, st, ghat, tw + twiddle_offset(L), lines, g.gy,
-                           g.Hx, dom);
+        hipLaunchKernelGGL((k_green_cols<L, 2>), dim3(nblocks(lines, Lines<L>::value), nj),
+                           dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
+                           g.Hx);
     }
 };
 
 template <typename R>
-void launch_green(const Geometry& g, int green_fn, const DomainRec* dom, const double2* tw64,
-                  double* ghat64, R* ghat_r, hipStream_t st)
+void launch_green(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
+                  const double2* tw64, hipStream_t st)
 {
-    dispatch_len<GreenX>(g.gx, g, green_fn, dom, tw64, ghat64, st);
-    dispatch_len<GreenY>(g.Dy, g, dom, tw64, ghat64, st);
-    dispatch_len<GreenZ>(g.Dz, g, dom, tw64, ghat64, st);
+    dispatch_len<GreenX>(g.gx, g, green_fn, jobs, nj, tw64, st);
+    dispatch_len<GreenY>(g.Dy, g, jobs, nj, tw64, st);
+    dispatch_len<GreenZ>(g.Dz, g, jobs, nj, tw64, st);
     if constexpr (!std::is_same<R, double>::value) {
-        hipLaunchKernelGGL(k_cast_ghat<R>, dim3(256), dim3(256), 0, st, ghat64, ghat_r,
-                           g.Ghat_elems, dom);
+        hipLaunchKernelGGL(k_cast_ghat<R>, dim3(256, nj), dim3(256), 0, st, jobs, g.Ghat_elems);
     }
 }
 
 template <typename R> struct SolveX {
+    using T2 = typename cplx<R>::type;
     template <int M> struct Fwd {
-        static void run(const Geometry& g, const R* rho, typename cplx<R>::type* S,
-                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
         {
             int rows = g.gz * g.gy;
-            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, Lines<M>::value)),
-                               dim3(kThreads), 0, st, rho, S, tw + twiddle_offset(M),
-                               tw + twiddle_offset(2 * M), rows, g.gy, g.Dy, dom);
+            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, Lines<M>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
+                               tw + twiddle_offset(2 * M), rows, g.gy, g.Dy);
         }
     };
     template <int M> struct Inv {
-        static void run(const Geometry& g, const typename cplx<R>::type* S, R* phi,
-                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
         {
             int rows = g.gz * g.gy;
-            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, Lines<M>::value)),
-                               dim3(kThreads), 0, st, S, phi, tw + twiddle_offset(M),
-                               tw + twiddle_offset(2 * M), rows, g.gy, g.Dy, dom);
+            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, Lines<M>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
+                               tw + twiddle_offset(2 * M), rows, g.gy, g.Dy);
         }
     };
     template <int L> struct YFwd {
-        static void run(const Geometry& g, typename cplx<R>::type* S,
-                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 0>), dim3(nblocks(lines, Lines<L>::value)),
-                               dim3(kThreads), 0, st, S, tw + twiddle_offset(L),
-                               (const R*)nullptr, lines, g.Hx, int64_t(g.Dy) * g.Hx,
-                               int64_t(g.Hx), g.gy, g.Dy, g.gy, g.Hx, dom);
+            hipLaunchKernelGGL((k_cols<R, L, 0>), dim3(nblocks(lines, Lines<L>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
+                               int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.gy, g.Dy, g.gy, g.Hx);
         }
     };
     template <int L> struct YInv {
-        static void run(const Geometry& g, typename cplx<R>::type* S,
-                        const typename cplx<R>::type* tw, const DomainRec* dom, hipStream_t st)
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 1>), dim3(nblocks(lines, Lines<L>::value)),
-                               dim3(kThreads), 0, st, S, tw + twiddle_offset(L),
-                               (const R*)nullptr, lines, g.Hx, int64_t(g.Dy) * g.Hx,
-                               int64_t(g.Hx), g.Dy, g.gy, g.gy, g.Hx, dom);
+            hipLaunchKernelGGL((k_cols<R, L, 1>), dim3(nblocks(lines, Lines<L>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
+                               int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.Dy, g.gy, g.gy, g.Hx);
         }
     };
     template <int L> struct ZConv {
-        static void run(const Geometry& g, typename cplx<R>::type* S,
-                        const typename cplx<R>::type* tw, const R* ghat, const DomainRec* dom,
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
                         hipStream_t st)
         {
             int lines = g.Dy * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 2>), dim3(nblocks(lines, Lines<L>::value)),
-                               dim3(kThreads), 0, st, S, tw + twiddle_offset(L), ghat, lines,
-                               lines, int64_t(0), int64_t(lines), g.gz, g.gz, g.gy, g.Hx, dom);
+            hipLaunchKernelGGL((k_cols<R, L, 2>), dim3(nblocks(lines, Lines<L>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, lines,
+                               int64_t(0), int64_t(lines), g.gz, g.gz, g.gy, g.Hx);
+        }
+    };
+    template <int L> struct ZField {
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
+        {
+            int lines = g.Dy * g.Hx;
+            hipLaunchKernelGGL((k_z_field<R, L>),
+                               dim3(nblocks(lines, FieldLines<L, 2, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gz,
+                               g.gy, g.Hx);
+        }
+    };
+    template <int L> struct YField {
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
+        {
+            int lines = g.gz * g.Hx;
+            hipLaunchKernelGGL((k_y_field<R, L>),
+                               dim3(nblocks(lines, FieldLines<L, 3, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
+                               g.Hx);
+        }
+    };
+    template <int M> struct XField {
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
+        {
+            int rows = g.gz * g.gy;
+            hipLaunchKernelGGL((k_x_field<R, M>),
+                               dim3(nblocks(rows, FieldLines<M, 3, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
+                               tw + twiddle_offset(2 * M), g.gy, g.gz, g.Dy);
         }
     };
 };
 
 template <typename R>
-void launch_solve(const Geometry& g, const R* rho, typename cplx<R>::type* S, const R* ghat,
-                  const typename cplx<R>::type* tw, R* phi, const DomainRec* dom, hipStream_t st)
+void launch_solve(const Geometry& g, const GridJob* jobs, int nj,
+                  const typename cplx<R>::type* tw, hipStream_t st)
 {
-    dispatch_len<SolveX<R>::template Fwd>(g.gx, g, rho, S, tw, dom, st);
-    dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, S, tw, dom, st);
-    dispatch_len<SolveX<R>::template ZConv>(g.Dz, g, S, tw, ghat, dom, st);
-    dispatch_len<SolveX<R>::template YInv>(g.Dy, g, S, tw, dom, st);
-    dispatch_len<SolveX<R>::template Inv>(g.gx, g, S, phi, tw, dom, st);
+    dispatch_len<SolveX<R>::template Fwd>(g.gx, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template ZConv>(g.Dz, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template YInv>(g.Dy, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template Inv>(g.gx, g, jobs, nj, tw, st);
 }
 
-template void launch_green<double>(const Geometry&, int, const DomainRec*, const double2*,
-                                   double*, double*, hipStream_t);
-template void launch_green<float>(const Geometry&, int, const DomainRec*, const double2*,
-                                  double*, float*, hipStream_t);
-template void launch_solve<double>(const Geometry&, const double*, double2*, const double*,
-                                   const double2*, double*, const DomainRec*, hipStream_t);
-template void launch_solve<float>(const Geometry&, const float*, float2*, const float*,
-                                  const float2*, float*, const DomainRec*, hipStream_t);
+template <typename R>
+void launch_solve_field(const Geometry& g, const GridJob* jobs, int nj,
+                        const typename cplx<R>::type* tw, hipStream_t st)
+{
+    dispatch_len<SolveX<R>::template Fwd>(g.gx, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template ZField>(g.Dz, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template YField>(g.Dy, g, jobs, nj, tw, st);
+    dispatch_len<SolveX<R>::template XField>(g.gx, g, jobs, nj, tw, st);
+}
+
+template void launch_green<double>(const Geometry&, int, const GridJob*, int, const double2*,
+                                   hipStream_t);
+template void launch_green<float>(const Geometry&, int, const GridJob*, int, const double2*,
+                                  hipStream_t);
+template void launch_solve<double>(const Geometry&, const GridJob*, int, const double2*,
+                                   hipStream_t);
+template void launch_solve_field<float>(const Geometry&, const GridJob*, int, const float2*,
+                                        hipStream_t);
 
 }  // namespace sc3d

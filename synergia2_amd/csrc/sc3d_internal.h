@@ -27,14 +27,40 @@ constexpr int64_t kTwiddleElems = 2 * kMaxLen - 1;
 
 bool fft_length_supported(int L);
 
-// Grid pipeline, FP64 or FP32 (R).  All launches on `st`, asynchronous.
-template <typename R>
-void launch_green(const Geometry& g, int green_fn, const DomainRec* dom,
-                  const double2* tw64, double* ghat64, R* ghat_r, hipStream_t st);
+template <typename R> struct vec4_of;
+template <> struct vec4_of<double> { using type = double4; };
+template <> struct vec4_of<float> { using type = float4; };
 
+// Per-job pointers of the grid pipeline.  The grid kernels take a device
+// table of these and serve job blockIdx.y, so one launch covers a batch of
+// same-shape jobs (a context alone is a table of one).
+struct GridJob {
+    void* rho;        // R[G] density octant
+    void* S;          // cplx<R>[S_elems] spectrum workspace
+    void* S2;         // precision form: second spectrum workspace
+    void* phi;        // FP64 form: R[G] potential octant
+    void* E;          // vec4<R>[G] field (Ex, Ey, Ez, 0)
+    double* ghat64;   // Green's spectrum octant, FP64
+    void* ghat;       // the convolution's copy (FP32 form), == ghat64 in FP64
+    DomainRec* dom;
+};
+
+// Green's spectrum (when the job's cell size changed) and, for R = float,
+// its FP32 copy.  All launches on `st`, asynchronous.
 template <typename R>
-void launch_solve(const Geometry& g, const R* rho, typename cplx<R>::type* S,
-                  const R* ghat, const typename cplx<R>::type* tw, R* phi,
-                  const DomainRec* dom, hipStream_t st);
+void launch_green(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
+                  const double2* tw64, hipStream_t st);
+
+// FP64 form: rho -> phi (five passes, convolution fused into the z pass).
+template <typename R>
+void launch_solve(const Geometry& g, const GridJob* jobs, int nj,
+                  const typename cplx<R>::type* tw, hipStream_t st);
+
+// Precision form: the same convolution, with the field transformed straight
+// from the spectrum (E = IDFT(phi^ * D_a), D_a = -i sin(2 pi k/D_a)/h_a) and
+// written as (Ex, Ey, Ez, 0) per node.  No potential is formed.
+template <typename R>
+void launch_solve_field(const Geometry& g, const GridJob* jobs, int nj,
+                        const typename cplx<R>::type* tw, hipStream_t st);
 
 }  // namespace sc3d

@@ -5,6 +5,7 @@
 
 #include "sc3d_common.h"
 #include "sc3d_maps.h"
+#include "sc3d_particles.h"
 
 namespace sc3d {
 
@@ -26,6 +27,41 @@ __global__ __launch_bounds__(kThreads) void k_maps(double* __restrict__ parts, i
         parts[i + 3 * stride] = yp;
         parts[i + 4 * stride] = cdt;
     }
+}
+
+// Served form: the scheduled program before the kick (first step only).
+__global__ __launch_bounds__(kThreads) void k_maps_sched(double* __restrict__ parts,
+                                                         int64_t stride,
+                                                         const uint8_t* __restrict__ masks,
+                                                         int64_t n, const MapProgram* progs,
+                                                         const StepCtl* ctl)
+{
+    if (ctl->done || ctl->first_prog < 0) return;
+    const MapProgram& prog = progs[ctl->first_prog];
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kThreads) {
+        if (!masks[i]) continue;
+        double x = parts[i], xp = parts[i + stride], y = parts[i + 2 * stride],
+               yp = parts[i + 3 * stride], cdt = parts[i + 4 * stride];
+        const double dpop = parts[i + 5 * stride];
+        for (int k = 0; k < prog.n; ++k)
+            apply_slice(prog.s[k], x, xp, y, yp, cdt, dpop, prog.ref_p, prog.mass);
+        parts[i] = x;
+        parts[i + stride] = xp;
+        parts[i + 2 * stride] = y;
+        parts[i + 3 * stride] = yp;
+        parts[i + 4 * stride] = cdt;
+    }
+}
+
+void launch_maps_sched(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                       const Sched& sc, hipStream_t st)
+{
+    int64_t blocks = (n + kThreads - 1) / kThreads;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) return;
+    hipLaunchKernelGGL(k_maps_sched, dim3(unsigned(blocks)), dim3(kThreads), 0, st, parts, stride,
+                       masks, n, sc.progs, sc.ctl);
 }
 
 }  // namespace sc3d

@@ -20,6 +20,14 @@ struct DomainFinishArgs {
     double time_step, ref_p, ref_gamma, ref_beta;
 };
 
+// Served form: the device-side step control of a context and its schedule
+// (all null for host-decided applies).
+struct Sched {
+    const StepCtl* ctl;
+    const MapProgram* progs;  // [first | between | last] x turn_steps
+    const double* dt;         // kick time step per schedule entry
+};
+
 struct PopulateArgs {
     double mean[6];
     double chol[21];
@@ -42,25 +50,37 @@ struct SortBuffers {
 constexpr int kSortTileItems = 4096;
 
 void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
-                   double* P, const DomainFinishArgs& a, DomainRec* dom, hipStream_t st);
+                   double* P, const DomainFinishArgs& a, DomainRec* dom, const Sched& sc,
+                   hipStream_t st);
+void launch_step_begin(StepCtl* ctl, hipStream_t st);
+// scheduled maps before the kick (the first step's first half-step)
+void launch_maps_sched(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                       const Sched& sc, hipStream_t st);
 int sort_passes(int64_t G);
 void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
                  int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st);
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
-                    double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st);
+                    double* scratch, uint8_t* smask, const DomainRec* dom, const StepCtl* ctl,
+                    hipStream_t st);
 template <typename R>
 void launch_fill(R* p, int64_t n, hipStream_t st);
+// FP64: k_deposit.  FP32 (precision form): k_deposit_f32 with unit weights,
+// then k_rho_peak records the density peak in the domain record.
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
-                    const int32_t* perm, int64_t n, int gx, int gy, int gz, const DomainRec* dom,
-                    R* rho, hipStream_t st);
+                    const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
+                    R* rho, const StepCtl* ctl, hipStream_t st);
+template <>
+void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* masks,
+                           const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
+                           float* rho, const StepCtl* ctl, hipStream_t st);
 template <typename R>
 void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
                    typename vec4<R>::type* E, hipStream_t st);
 template <typename R>
 void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
                  int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
-                 double m, const MapProgram& prog, hipStream_t st);
+                 double m, const MapProgram& prog, const Sched& sc, hipStream_t st);
 void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                     double total_num, double* P, double* out, hipStream_t st);
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,

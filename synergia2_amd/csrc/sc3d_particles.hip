@@ -6,6 +6,36 @@
 namespace sc3d {
 
 // ---------------------------------------------------------------------------
+// Served form: opens one apply of a scheduled job (see StepCtl).  Schedule
+// table layout: [first | between | last], turn_steps programs each; step j
+// of the job is schedule entry k = j mod turn_steps; the maps before the
+// kick run on the first step only, and the program fused after the kick is
+// between[k] (the second half-step of k and the first of k+1) except after
+// the last step (last[k]).
+// ---------------------------------------------------------------------------
+__global__ void k_step_begin(StepCtl* __restrict__ ctl)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    StepCtl c = *ctl;
+    const int64_t j = c.applies;
+    if (c.turn_steps > 0 && j >= c.total_steps) {
+        c.done = 1;
+        *ctl = c;
+        return;
+    }
+    c.done = 0;
+    c.reorder = c.sort_period > 0 && (j % c.sort_period) == 0;
+    if (c.turn_steps > 0) {
+        const int k = int(j % c.turn_steps);
+        c.cur = k;
+        c.first_prog = j == 0 ? k : -1;
+        c.after_prog = j < c.total_steps - 1 ? c.turn_steps + k : 2 * c.turn_steps + k;
+    }
+    c.applies = j + 1;
+    *ctl = c;
+}
+
+// ---------------------------------------------------------------------------
 // Bunch extent: the six sums of Core_diagnostics::calculate_spatial_mean_stddev
 // (core_diagnostics.cc:139-153) in one fixed tree, identical for every launch
 // configuration and device:
@@ -132,9 +162,11 @@ void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, i
 __global__ __launch_bounds__(kThreads) void k_domain_partial(const double* __restrict__ parts,
                                                              int64_t stride,
                                                              const uint8_t* __restrict__ masks,
-                                                             int64_t n, double* __restrict__ P)
+                                                             int64_t n, double* __restrict__ P,
+                                                             const StepCtl* __restrict__ ctl)
 {
     __shared__ double ws[4][6];
+    if (ctl && ctl->done) return;
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int64_t c = blockIdx.x;
 #pragma unroll
@@ -171,9 +203,18 @@ __device__ double smallest_non_tiny(double val, double o1, double o2, double tin
 // bits) or must be recomputed this step.
 __global__ __launch_bounds__(kThreads) void k_domain_finish(const double* __restrict__ P,
                                                             int64_t nch, DomainFinishArgs a,
-                                                            DomainRec* __restrict__ dom)
+                                                            DomainRec* __restrict__ dom, Sched sc)
 {
     __shared__ double ws[4][6];
+    if (sc.ctl) {
+        // served form: a finished job's applies do nothing; the kick time
+        // step comes from the schedule
+        if (sc.ctl->done) {
+            if (threadIdx.x == 0) dom->error = kDoneMark;
+            return;
+        }
+        if (sc.ctl->turn_steps > 0) a.time_step = sc.dt[sc.ctl->cur];
+    }
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (!a.fixed) {
         for (int64_t c = threadIdx.x; c < nch; c += kThreads)
@@ -185,6 +226,7 @@ __global__ __launch_bounds__(kThreads) void k_domain_finish(const double* __rest
     DomainRec d = *dom;  // keeps the Green's cache fields
     double size[3], offset[3];
     d.error = 0;
+    d.rho_peak = 0;
     if (a.fixed) {
         for (int k = 0; k < 3; ++k) {
             size[k] = a.size[k];
@@ -427,9 +469,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 __global__ __launch_bounds__(kThreads) void k_gather_particles(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
     const int32_t* __restrict__ perm, int64_t n, double* __restrict__ scratch,
-    uint8_t* __restrict__ smask, const DomainRec* dom)
+    uint8_t* __restrict__ smask, const DomainRec* dom, const StepCtl* __restrict__ ctl)
 {
-    if (dom->error) return;
+    if (dom->error || (ctl && !ctl->reorder)) return;
     for (int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x; j < n;
          j += int64_t(gridDim.x) * kThreads) {
         int64_t src = perm[j];
@@ -444,9 +486,10 @@ __global__ __launch_bounds__(kThreads) void k_copy_back(double* __restrict__ par
                                                         uint8_t* __restrict__ masks, int64_t n,
                                                         const double* __restrict__ scratch,
                                                         const uint8_t* __restrict__ smask,
-                                                        const DomainRec* dom)
+                                                        const DomainRec* dom,
+                                                        const StepCtl* __restrict__ ctl)
 {
-    if (dom->error) return;
+    if (dom->error || (ctl && !ctl->reorder)) return;
     for (int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x; j < n;
          j += int64_t(gridDim.x) * kThreads) {
 #pragma unroll
@@ -483,12 +526,15 @@ template <typename R>
 __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__ parts,
                                                       int64_t stride,
                                                       const uint8_t* __restrict__ masks,
-                                                      const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ perm_in,
                                                       int64_t n, int gx, int gy, int gz,
                                                       const DomainRec* __restrict__ dom,
-                                                      R* __restrict__ rho)
+                                                      R* __restrict__ rho,
+                                                      const StepCtl* __restrict__ ctl)
 {
     if (dom->error) return;
+    // served form: right after a reorder the bunch itself is in cell order
+    const int32_t* __restrict__ perm = (ctl && ctl->reorder) ? nullptr : perm_in;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
     const double w0 = dom->w0;
@@ -555,6 +601,118 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Precision-form deposit (FP32 grid).  The cell index and offsets come from
+// the FP64 coordinates exactly as above; the eight corner weights are unit
+// charge weights aox*aoy*aoz (left to right) rounded to FP32, and the
+// per-wavefront segment sums are compensated FP32 (TwoSum: every lane carries
+// a sum and its rounding error, combined error-free), so a segment's total is
+// its exact sum rounded once.  One FP32 atomic per segment tail and corner.
+// The physical weight w0 and the unit-peak scale return in FP64 at the kick.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__global__ __launch_bounds__(kThreads) void k_deposit_f32(const double* __restrict__ parts,
+                                                          int64_t stride,
+                                                          const uint8_t* __restrict__ masks,
+                                                          const int32_t* __restrict__ perm_in,
+                                                          int64_t n, int gx, int gy, int gz,
+                                                          const DomainRec* __restrict__ dom,
+                                                          float* __restrict__ rho,
+                                                          const StepCtl* __restrict__ ctl)
+{
+    if (dom->error) return;
+    const int32_t* __restrict__ perm = (ctl && ctl->reorder) ? nullptr : perm_in;
+    const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
+    const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
+    const int lane = threadIdx.x & 63;
+    const uint64_t le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
+    const int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+    int64_t i = j;
+    if (perm && j < n) {
+        i = perm[j];
+        if (i < 0 || i >= n) i = j;  // perm is a permutation of [0, n)
+    }
+    int ix = 0, iy = 0, iz = 0;
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    bool ok = false;
+    if (j < n && masks[i]) {
+        ix = cell_index(parts[i], lx, ihx, ox);
+        iy = cell_index(parts[i + 2 * stride], ly, ihy, oy);
+        iz = cell_index(parts[i + 4 * stride], lz, ihz, oz);
+        ok = ix >= 0 && ix <= gx - 2 && iy >= 0 && iy <= gy - 2 && iz >= 0 && iz <= gz - 2;
+    }
+    const int key = ok ? (iz * gy + iy) * gx + ix : -1 - lane;  // lanes out: singletons
+    float w[8], e[8];
+    if (ok) {
+        const float fx = float(ox), fy = float(oy), fz = float(oz);
+        const float ax = float(1.0 - ox), ay = float(1.0 - oy), az = float(1.0 - oz);
+        w[0] = ax * ay * az;
+        w[1] = fx * ay * az;
+        w[2] = ax * fy * az;
+        w[3] = fx * fy * az;
+        w[4] = ax * ay * fz;
+        w[5] = fx * ay * fz;
+        w[6] = ax * fy * fz;
+        w[7] = fx * fy * fz;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w[c] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) e[c] = 0.0f;
+    const int kprev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    const int seg = __popcll(__ballot(head) & le_mask);
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const int seg_up = __shfl_up(seg, s, 64);
+        const bool same = (lane >= s) & (seg_up == seg);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float v = __shfl_up(w[c], s, 64);
+            const float ve = __shfl_up(e[c], s, 64);
+            if (same) {
+                float t, err;
+                two_sum(w[c], v, t, err);
+                w[c] = t;
+                e[c] = (e[c] + ve) + err;
+            }
+        }
+    }
+    const int seg_down = __shfl_down(seg, 1, 64);
+    const bool tail = (lane == 63) | (seg_down != seg);
+    if (ok && tail) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1)
+                unsafeAtomicAdd(rho + (int64_t(cz) * gy + cy) * gx + cx, w[c] + e[c]);
+        }
+    }
+}
+
+// Peak of the unit-weight FP32 density (all entries >= 0, so the IEEE bit
+// patterns order like the values): wavefront max, one integer atomicMax per
+// wavefront into the domain record (cleared by k_domain_finish each apply).
+__global__ __launch_bounds__(kThreads) void k_rho_peak(const float* __restrict__ rho, int64_t G,
+                                                       DomainRec* __restrict__ dom)
+{
+    if (dom->error) return;
+    float m = 0.0f;
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < G;
+         i += int64_t(gridDim.x) * kThreads)
+        m = fmaxf(m, rho[i]);
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2) m = fmaxf(m, __shfl_xor(m, w, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(&dom->rho_peak, __float_as_int(m));
+}
+
+// ---------------------------------------------------------------------------
 // E field (zyx::alg_force_extractor_doubled_domain, space_charge_3d_kernels.h:
 // 217-307): E = -(phi[+1] - phi[-1]) * (0.5/h) on interior nodes, 0 on the
 // boundary.  Stored interleaved (Ex, Ey, Ez, 0) per node for the kick's
@@ -594,18 +752,32 @@ __global__ __launch_bounds__(kThreads) void k_efield(const R* __restrict__ phi, 
 // sequence is applied to every active particle in the same pass (one read
 // and one write of the six coordinates for kick + maps).
 // ---------------------------------------------------------------------------
+// Kick factor: the FP64 form's is the record's; the precision form's grid
+// holds unit-weight charge scaled to unit peak, so w0 * peak returns here.
+template <typename R>
+__device__ __forceinline__ double kick_factor(const DomainRec& d)
+{
+    if (sizeof(R) == sizeof(double)) return d.factor;
+    const float peak = __int_as_float(d.rho_peak);
+    return d.factor * d.w0 * (peak > 0.0f ? double(peak) : 1.0);
+}
+
 template <typename R, bool MAPS>
 __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, int64_t stride,
                                                    const uint8_t* __restrict__ masks, int64_t n,
                                                    int gx, int gy, int gz,
                                                    const DomainRec* __restrict__ dom,
                                                    const typename vec4<R>::type* __restrict__ E,
-                                                   double pref, double m, MapProgram prog)
+                                                   double pref, double m, MapProgram prog,
+                                                   Sched sc)
 {
     if (dom->error) return;
+    // served form: the program fused after the kick comes from the schedule
+    const MapProgram* __restrict__ sp = (MAPS && sc.progs) ? sc.progs + sc.ctl->after_prog
+                                                            : nullptr;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
-    const double factor = dom->factor;
+    const double factor = kick_factor<R>(*dom);
     for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kThreads) {
         if (!masks[i]) continue;
@@ -643,8 +815,13 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
             xp += factor * vx;
         }
         if (MAPS) {
-            for (int k = 0; k < prog.n; ++k)
-                apply_slice(prog.s[k], x, xp, y, yp, cdt, dpop, prog.ref_p, prog.mass);
+            if (sp) {
+                for (int k = 0; k < sp->n; ++k)
+                    apply_slice(sp->s[k], x, xp, y, yp, cdt, dpop, sp->ref_p, sp->mass);
+            } else {
+                for (int k = 0; k < prog.n; ++k)
+                    apply_slice(prog.s[k], x, xp, y, yp, cdt, dpop, prog.ref_p, prog.mass);
+            }
             parts[i] = x;
             parts[i + 2 * stride] = y;
             parts[i + 4 * stride] = cdt;
@@ -722,14 +899,20 @@ static int grid_for(int64_t n, int cap = 8192)
 }
 
 void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
-                   double* P, const DomainFinishArgs& a, DomainRec* dom, hipStream_t st)
+                   double* P, const DomainFinishArgs& a, DomainRec* dom, const Sched& sc,
+                   hipStream_t st)
 {
     int64_t nch = (n + 4095) / 4096;
     if (!a.fixed && nch > 0)
         hipLaunchKernelGGL(k_domain_partial, dim3(unsigned(nch)), dim3(kThreads), 0, st, parts,
-                           stride, masks, n, P);
+                           stride, masks, n, P, sc.ctl);
     hipLaunchKernelGGL(k_domain_finish, dim3(1), dim3(kThreads), 0, st, P, a.fixed ? 0 : nch, a,
-                       dom);
+                       dom, sc);
+}
+
+void launch_step_begin(StepCtl* ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, st, ctl);
 }
 
 int sort_passes(int64_t G)
@@ -781,12 +964,13 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
 }
 
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
-                    double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st)
+                    double* scratch, uint8_t* smask, const DomainRec* dom, const StepCtl* ctl,
+                    hipStream_t st)
 {
     hipLaunchKernelGGL(k_gather_particles, dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                       stride, masks, perm, n, scratch, smask, dom);
+                       stride, masks, perm, n, scratch, smask, dom, ctl);
     hipLaunchKernelGGL(k_copy_back, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride,
-                       masks, n, scratch, smask, dom);
+                       masks, n, scratch, smask, dom, ctl);
 }
 
 template <typename R>
@@ -797,13 +981,26 @@ void launch_fill(R* p, int64_t n, hipStream_t st)
 
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
-                    const int32_t* perm, int64_t n, int gx, int gy, int gz, const DomainRec* dom,
-                    R* rho, hipStream_t st)
+                    const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
+                    R* rho, const StepCtl* ctl, hipStream_t st)
 {
     int64_t nb = (n + kThreads - 1) / kThreads;
     if (nb < 1) return;
     hipLaunchKernelGGL(k_deposit<R>, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts, stride,
-                       masks, perm, n, gx, gy, gz, dom, rho);
+                       masks, perm, n, gx, gy, gz, dom, rho, ctl);
+}
+
+template <>
+void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* masks,
+                           const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
+                           float* rho, const StepCtl* ctl, hipStream_t st)
+{
+    int64_t nb = (n + kThreads - 1) / kThreads;
+    if (nb >= 1)
+        hipLaunchKernelGGL(k_deposit_f32, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
+                           stride, masks, perm, n, gx, gy, gz, dom, rho, ctl);
+    const int64_t G = int64_t(gx) * gy * gz;
+    hipLaunchKernelGGL(k_rho_peak, dim3(grid_for(G, 1024)), dim3(kThreads), 0, st, rho, G, dom);
 }
 
 template <typename R>
@@ -817,14 +1014,14 @@ void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
 template <typename R>
 void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
                  int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
-                 double m, const MapProgram& prog, hipStream_t st)
+                 double m, const MapProgram& prog, const Sched& sc, hipStream_t st)
 {
-    if (prog.n > 0)
+    if (prog.n > 0 || sc.progs)
         hipLaunchKernelGGL((k_kick<R, true>), dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog);
+                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog, sc);
     else
         hipLaunchKernelGGL((k_kick<R, false>), dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog);
+                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog, sc);
 }
 
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
@@ -837,20 +1034,17 @@ void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
 template void launch_fill<double>(double*, int64_t, hipStream_t);
 template void launch_fill<float>(float*, int64_t, hipStream_t);
 template void launch_deposit<double>(const double*, int64_t, const uint8_t*, const int32_t*,
-                                     int64_t, int, int, int, const DomainRec*, double*,
+                                     int64_t, int, int, int, DomainRec*, double*, const StepCtl*,
                                      hipStream_t);
-template void launch_deposit<float>(const double*, int64_t, const uint8_t*, const int32_t*,
-                                    int64_t, int, int, int, const DomainRec*, float*,
-                                    hipStream_t);
 template void launch_efield<double>(const double*, int, int, int, const DomainRec*, double4*,
                                     hipStream_t);
 template void launch_efield<float>(const float*, int, int, int, const DomainRec*, float4*,
                                    hipStream_t);
 template void launch_kick<double>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
                                   const DomainRec*, const double4*, double, double,
-                                  const MapProgram&, hipStream_t);
+                                  const MapProgram&, const Sched&, hipStream_t);
 template void launch_kick<float>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
                                  const DomainRec*, const float4*, double, double,
-                                 const MapProgram&, hipStream_t);
+                                 const MapProgram&, const Sched&, hipStream_t);
 
 }  // namespace sc3d

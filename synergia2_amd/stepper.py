@@ -84,10 +84,12 @@ def periodic_twiss(M):
 
 def matched_chol(elements_one_cell, emit, sigma_cdt, sigma_dpop):
     """Packed 6x6 lower-triangular factor of a Gaussian matched to the cell
-    (uncoupled planes, rms geometric emittance emit)."""
+    (uncoupled planes, rms geometric emittance emit, or (emit_x, emit_y))."""
     L = np.zeros(21)
+    emits = tuple(emit) if np.ndim(emit) else (emit, emit)
     for plane in (0, 1):
         beta, alpha, _ = periodic_twiss(cell_matrix(elements_one_cell, plane))
+        emit = emits[plane]
         r0 = 2 * plane
         a = math.sqrt(emit * beta)
         L[r0 * (r0 + 1) // 2 + r0] = a
@@ -97,6 +99,15 @@ def matched_chol(elements_one_cell, emit, sigma_cdt, sigma_dpop):
     L[4 * 5 // 2 + 4] = sigma_cdt
     L[5 * 6 // 2 + 5] = sigma_dpop
     return L
+
+
+def matched_chol_sigma(elements_one_cell, sigma_x, sigma_y, sigma_cdt, sigma_dpop):
+    """Gaussian matched to the cell with the given rms beam sizes at the cell
+    entrance: emit_plane = sigma_plane^2 / beta_plane."""
+    bx = periodic_twiss(cell_matrix(elements_one_cell, 0))[0]
+    by = periodic_twiss(cell_matrix(elements_one_cell, 1))[0]
+    return matched_chol(elements_one_cell, (sigma_x ** 2 / bx, sigma_y ** 2 / by), sigma_cdt,
+                        sigma_dpop)
 
 
 class Propagator:
