@@ -98,6 +98,10 @@ def main():
     ap.add_argument("--accuracy", action="store_true",
                     help="also state field rel-L2 and rms-emittance drift of the FP32-grid form "
                          "against the FP64 form and the oracle on this workload")
+    ap.add_argument("--served", action="store_true",
+                    help="measure the served form on the Poisson job stream instead (see "
+                         "served_bench)")
+    ap.add_argument("--jobs", type=int, default=256, help="served: jobs in the stream")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a 1 GiB device-to-device copy (counter calibration and "
                          "achievable HBM bandwidth)")
@@ -113,6 +117,10 @@ def main():
 
     import synergia2_amd as sc
     from synergia2_amd import stepper as S
+
+    if args.served:
+        served_bench(args, sc, S)
+        return
 
     copy_gbps = copy_probe(torch) if args.copy_probe else None
 
@@ -208,6 +216,99 @@ def main():
     }
     if acc is not None:
         out["accuracy"] = acc
+    print(json.dumps(out), flush=True)
+
+
+def served_bench(args, sc, S):
+    """Served form on the job stream: Poisson arrivals (mean 20 jobs/s) of
+    `--jobs` FODO-channel jobs (seed 4; particles log-uniform in [1e4, 4e6];
+    grid from {32x32x64, 64x64x128, 128x128x256}; steps uniform in [20, 200];
+    sigma_x, sigma_y uniform in [0.5, 3] mm (matched), sigma_z in [5, 30] mm),
+    every job FP32 grid / FP64 particles.
+      stream:    jobs submitted at their arrival times (real time), server
+                 iterating; per-step (iteration) and per-job (submit to done)
+                 latency percentiles; throughput = particle-steps / makespan
+      saturated: the same jobs all submitted at t = 0; throughput =
+                 particle-steps / wall time (value)
+      fp64_plain: the same jobs one after another through the operator API
+                 in the plain FP64 form (Propagator, fused maps); throughput
+                 = particle-steps / wall time."""
+    import numpy as np
+    import torch
+    from synergia2_amd import serve as SV
+
+    stream = SV.job_stream(count=args.jobs, seed=4)
+    total = sum(kw["n"] * kw["steps"] for _, kw in stream)
+    print(f"# served: {len(stream)} jobs, {total:.4e} particle-steps, last arrival "
+          f"{stream[-1][0]:.2f} s", file=sys.stderr, flush=True)
+
+    def drive(real_time):
+        srv = SV.Server(SV.options())
+        pending = list(stream)
+        ids = []
+        t0 = time.perf_counter()
+        while True:
+            now = time.perf_counter() - t0
+            while pending and (not real_time or pending[0][0] <= now):
+                ids.append(srv.submit(SV.Job(**pending.pop(0)[1])))
+            res, wait = srv.iterate()
+            if res == 0 and wait == 0:
+                if not pending:
+                    break
+                time.sleep(max(0.0, min(0.005, pending[0][0] - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        lat = [srv.result(i).t_done - srv.result(i).t_submit for i in ids]
+        srv.close()
+        torch.cuda.empty_cache()
+        return wall, st, lat
+
+    wall_s, st_s, lat_s = drive(True)
+    print(f"# stream: wall {wall_s:.2f} s, {st_s['iterations']} iterations, "
+          f"{st_s['graph_builds']} graph builds, step p50/p99 {st_s['iter_ms_p50']:.3f}/"
+          f"{st_s['iter_ms_p99']:.3f} ms, job p50/p99 {np.percentile(lat_s, 50):.3f}/"
+          f"{np.percentile(lat_s, 99):.3f} s", file=sys.stderr, flush=True)
+    wall_x, st_x, lat_x = drive(False)
+    print(f"# saturated: wall {wall_x:.2f} s -> {total / wall_x:.4e} particle-steps/s, "
+          f"{st_x['iterations']} iterations, {st_x['graph_builds']} graph builds",
+          file=sys.stderr, flush=True)
+
+    # the plain FP64 form on the same jobs, one at a time
+    t0 = time.perf_counter()
+    for _, kw in stream:
+        job = SV.Job(**kw)
+        b = sc.Bunch(job.n, job.real_num, gamma=job.gamma)
+        b.populate_gaussian(np.zeros(6), job.chol, seed=job.seed)
+        op = sc.SpaceCharge3dOpenHockney(sc.default_options(*job.shape))
+        S.Propagator(job.lattice, job.nsteps, b, op).run(job.steps, sync=True)
+        op.close()
+        del b
+    torch.cuda.synchronize()
+    wall_64 = time.perf_counter() - t0
+    print(f"# fp64 plain: wall {wall_64:.2f} s -> {total / wall_64:.4e} particle-steps/s",
+          file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "particle-kicks/s (served FP32-grid form, Poisson job mix)",
+        "value": total / wall_x,
+        "unit": "particle-kicks/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32-grid/f64-particles",
+        "data": "synthetic (seeded job stream)",
+        "config": {"workload": f"{len(stream)} FODO jobs, Poisson 20/s, seed 4",
+                   "particle_steps": total},
+        "saturated": {"wall_s": wall_x, "iterations": st_x["iterations"],
+                      "graph_builds": st_x["graph_builds"],
+                      "step_ms_p50": st_x["iter_ms_p50"], "step_ms_p99": st_x["iter_ms_p99"]},
+        "stream": {"wall_s": wall_s, "throughput": total / wall_s,
+                   "iterations": st_s["iterations"], "graph_builds": st_s["graph_builds"],
+                   "step_ms_p50": st_s["iter_ms_p50"], "step_ms_p99": st_s["iter_ms_p99"],
+                   "job_s_p50": float(np.percentile(lat_s, 50)),
+                   "job_s_p99": float(np.percentile(lat_s, 99))},
+        "fp64_plain": {"wall_s": wall_64, "throughput": total / wall_64},
+        "vs_fp64_plain": (total / wall_x) / (total / wall_64),
+    }
     print(json.dumps(out), flush=True)
 
 

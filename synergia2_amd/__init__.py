@@ -124,7 +124,11 @@ PHASES = ("sc3d_domain", "sc3d_sort", "sc3d_local_rho", "sc3d_green_fn2", "sc3d_
 EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
            "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
-           "sc3d_moments")
+           "sc3d_moments",
+           # served form (include/sc3d_serve.h)
+           "sc3d_server_options_default", "sc3d_server_create", "sc3d_server_submit",
+           "sc3d_server_iterate", "sc3d_server_job", "sc3d_server_job_particles",
+           "sc3d_server_get_stats", "sc3d_server_destroy", "sc3d_job_footprint")
 
 _lib = None
 

@@ -1,5 +1,5 @@
 """Host-side logic that needs no GPU: the C ABI library loads and exports
-every entry point include/sc3d_capi.h declares; option defaults mirror the
+every entry point include/*.h declares; option defaults mirror the
 reference; the split-operator slicing and the matched FODO beam."""
 import math
 import os
@@ -15,9 +15,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    txt = open(os.path.join(ROOT, "include", "sc3d_capi.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(sc3d_[a-z0-9_]+)\s*\(", txt)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, f)).read(), flags=re.S)
+            names |= set(re.findall(r"\b(sc3d_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_capi_exports_every_declared_symbol():
@@ -76,3 +80,20 @@ def test_matched_fodo_beam():
     # one period maps the matched ellipse onto itself
     M = S.cell_matrix(cell, 0)
     np.testing.assert_allclose(M @ cov[:2, :2] @ M.T, cov[:2, :2], rtol=1e-10)
+
+
+def test_job_footprint_host_function():
+    """sc3d_job_footprint: the admission size of a served job grows with the
+    particle count and the grid, and covers at least its particles and
+    its spectrum workspaces (no device needed)."""
+    from synergia2_amd import serve as SV
+    small = SV.Job(10000, (32, 32, 64), 20, 1e-3, 1e-3, 0.01, seed=1)
+    big_n = SV.Job(1000000, (32, 32, 64), 20, 1e-3, 1e-3, 0.01, seed=1)
+    big_g = SV.Job(10000, (128, 128, 256), 20, 1e-3, 1e-3, 0.01, seed=1)
+    f = [j.footprint() for j in (small, big_n, big_g)]
+    assert f[0] < f[1] and f[0] < f[2]
+    n, (gx, gy, gz) = 1000000, (32, 32, 64)
+    assert f[1] >= n * (7 * 8 + 1)
+    s_elems = gz * 2 * gy * (gx + 1)
+    assert f[0] >= 2 * s_elems * 8  # two FP32 spectrum workspaces
+    assert SV.Job(10000, (32, 32, 64), 20, 1e-3, 1e-3, 0.01, seed=1).footprint(sc.FP64) > 0
