@@ -102,6 +102,11 @@ def main():
                     help="measure the served form on the Poisson job stream instead (see "
                          "served_bench)")
     ap.add_argument("--jobs", type=int, default=256, help="served: jobs in the stream")
+    ap.add_argument("--long", action="store_true",
+                    help="long tracking of the precision form against the FP64 form on the "
+                         "FODO ring (see long_tracking)")
+    ap.add_argument("--turns", type=int, default=200, help="long: turns")
+    ap.add_argument("--long-particles", type=int, default=10_000_000, help="long: particles")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a 1 GiB device-to-device copy (counter calibration and "
                          "achievable HBM bandwidth)")
@@ -120,6 +125,9 @@ def main():
 
     if args.served:
         served_bench(args, sc, S)
+        return
+    if args.long:
+        long_tracking(args, sc, S)
         return
 
     copy_gbps = copy_probe(torch) if args.copy_probe else None
@@ -216,6 +224,83 @@ def main():
     }
     if acc is not None:
         out["accuracy"] = acc
+    print(json.dumps(out), flush=True)
+
+
+def long_tracking(args, sc, S):
+    """Long tracking: 1e7 macroparticles, 90 % 6D Gaussian matched to the
+    cell (sigma_x = sigma_y = 2 mm, sigma_z = 20 mm, sigma_dp/p = 1e-4) + 10 %
+    uniform in the transverse 4D ellipsoid at 3 sigma (seed 3; population
+    2e11), through a ring of 32 O-F-O-D cells with one kick per element (128
+    kicks per turn, Split_operator_stepper_elements(1)), grid 128x128x256.
+    The precision form (FP32 grid) and the FP64 form track the same bunch;
+    after every turn both report rms emittances and the fraction of
+    particles beyond 4 initial sigma (x, y).  The FP64 form stands in for
+    the oracle here (it matches the oracle within 1e-12 per step; 25,600
+    oracle steps of 1e7 particles would take hours on the host)."""
+    import numpy as np
+    import torch
+
+    n, turns = args.long_particles, args.turns
+    gamma = (sc.PROTON_MASS + KINETIC_ENERGY) / sc.PROTON_MASS
+    beta, _ = sc.kinematics(sc.PROTON_MASS, gamma)
+    ring = S.fodo_channel(32)
+    steps = S.split_operator_steps_elements(ring, 1)
+    chol = S.matched_chol_sigma(S.fodo_channel(1), 2e-3, 2e-3, 0.020 / beta, 1e-4)
+    parts = S.halo_mix_parts(n, chol, seed=3)
+    forms = {}
+    for prec in (sc.FP32_GRID, sc.FP64):
+        b = sc.Bunch(n, 2e11, gamma=gamma)
+        b.set_host(parts)
+        opts = sc.default_options(128, 128, 256)
+        opts.precision = prec
+        op = sc.SpaceCharge3dOpenHockney(opts)
+        forms[prec] = (b, op, S.Propagator(ring, len(steps), b, op, steps=steps))
+    del parts
+    mean0, mom0 = forms[sc.FP64][1].moments(forms[sc.FP64][0])
+    width0 = np.sqrt(np.diag(mom0))
+    e0 = sc.emittances(mom0)
+    hist = {sc.FP32_GRID: [], sc.FP64: []}
+    tsec = {sc.FP32_GRID: 0.0, sc.FP64: 0.0}
+    print(f"# long: {n} particles, {turns} turns x {len(steps)} kicks, emit0 {e0}",
+          file=sys.stderr, flush=True)
+    for t in range(1, turns + 1):
+        for prec, (b, op, prop) in forms.items():
+            t0 = time.perf_counter()
+            prop.run(len(steps), sync=True)
+            tsec[prec] += time.perf_counter() - t0
+            mean, mom2 = op.moments(b)
+            tail = op.tail_fraction(b, mean, width0, 4.0)
+            hist[prec].append((sc.emittances(mom2), [float(tail[0]), float(tail[2])]))
+        if t % 10 == 0 or t == turns:
+            e32, tl32 = hist[sc.FP32_GRID][-1]
+            e64, tl64 = hist[sc.FP64][-1]
+            print(f"# turn {t}: emit32/emit64-1 = "
+                  f"{[f'{e32[k] / e64[k] - 1:.2e}' for k in range(3)]}, "
+                  f"tail x,y fp32 {tl32} fp64 {tl64}, "
+                  f"s/turn fp32 {tsec[sc.FP32_GRID] / t:.3f} fp64 {tsec[sc.FP64] / t:.3f}",
+                  file=sys.stderr, flush=True)
+    drift = [max(abs(hist[sc.FP32_GRID][i][0][k] / hist[sc.FP64][i][0][k] - 1)
+                 for i in range(turns)) for k in range(3)]
+    tail_diff = [max(abs(hist[sc.FP32_GRID][i][1][k] - hist[sc.FP64][i][1][k])
+                     for i in range(turns)) for k in range(2)]
+    out = {
+        "metric": "long tracking: precision form vs FP64 form, per turn",
+        "config": {"workload": "32-cell FODO ring, 1 kick per element, 90% Gaussian + 10% "
+                               "3-sigma 4D ellipsoid halo, seed 3",
+                   "particles": n, "turns": turns, "kicks_per_turn": len(steps),
+                   "grid": [128, 128, 256]},
+        "emittance_rel_drift_max": drift,
+        "tail4sigma_abs_diff_max": tail_diff,
+        "final": {"emit_fp32": hist[sc.FP32_GRID][-1][0], "emit_fp64": hist[sc.FP64][-1][0],
+                  "tail_fp32": hist[sc.FP32_GRID][-1][1], "tail_fp64": hist[sc.FP64][-1][1]},
+        "emit0": e0,
+        "ms_per_step": {"fp32_grid": 1e3 * tsec[sc.FP32_GRID] / (turns * len(steps)),
+                        "fp64": 1e3 * tsec[sc.FP64] / (turns * len(steps))},
+        "per_turn": [{"turn": i + 1, "emit_fp32": hist[sc.FP32_GRID][i][0],
+                      "emit_fp64": hist[sc.FP64][i][0], "tail_fp32": hist[sc.FP32_GRID][i][1],
+                      "tail_fp64": hist[sc.FP64][i][1]} for i in range(turns)],
+    }
     print(json.dumps(out), flush=True)
 
 

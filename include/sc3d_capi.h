@@ -211,6 +211,14 @@ int sc3d_apply_maps(const sc3d_bunch* bunch, const sc3d_map_slice* slices, int n
 int sc3d_moments(sc3d_ctx* ctx, const sc3d_bunch* bunch, double mean[6], double mom2[36],
                  void* stream);
 
+/* Tail fractions: per plane j, the active particles with
+ * |coordinate_j - center[j]| > nsigma * width[j], divided by total_num
+ * (halo diagnostic of long tracking, e.g. beyond 4 sigma; no counterpart in
+ * the reference's diagnostics).  Integer counts, so independent of the
+ * reduction order.  Synchronises `stream`. */
+int sc3d_tail_fraction(sc3d_ctx* ctx, const sc3d_bunch* bunch, const double center[6],
+                       const double width[6], double nsigma, double frac[6], void* stream);
+
 /* Fills slots [0, n) of a bunch: coordinates = mean + L * u with u a
  * standard normal 6-vector from Philox-4x32-10 keyed (seed, id) and
  * Box-Muller, L lower-triangular packed row-major (21 entries); column 6 =

@@ -124,7 +124,7 @@ PHASES = ("sc3d_domain", "sc3d_sort", "sc3d_local_rho", "sc3d_green_fn2", "sc3d_
 EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
            "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
-           "sc3d_moments",
+           "sc3d_moments", "sc3d_tail_fraction",
            # served form (include/sc3d_serve.h)
            "sc3d_server_options_default", "sc3d_server_create", "sc3d_server_submit",
            "sc3d_server_iterate", "sc3d_server_job", "sc3d_server_job_particles",
@@ -163,10 +163,13 @@ def load(path=None):
                                C.POINTER(C.c_double), C.c_void_p]
     L.sc3d_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.c_int]
+    L.sc3d_tail_fraction.argtypes = [C.c_void_p, C.POINTER(BunchDesc), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.c_double,
+                                     C.POINTER(C.c_double), C.c_void_p]
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
                  "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
-                 "sc3d_get_timing", "sc3d_set_post_kick_maps",
-           "sc3d_moments"):
+                 "sc3d_get_timing", "sc3d_set_post_kick_maps", "sc3d_moments",
+                 "sc3d_tail_fraction"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L
@@ -321,6 +324,20 @@ class SpaceCharge3dOpenHockney:
         _check(load().sc3d_moments(self._ctx, C.byref(d), mean.ctypes.data_as(C.POINTER(C.c_double)),
                                    mom2.ctypes.data_as(C.POINTER(C.c_double)), _stream_ptr(stream)))
         return mean, mom2.reshape(6, 6)
+
+    def tail_fraction(self, bunch, center, width, nsigma=4.0, stream=None):
+        """Per-plane fraction of active particles with |p - center| >
+        nsigma * width (device counts, exact)."""
+        c = np.ascontiguousarray(center, np.float64)
+        w = np.ascontiguousarray(width, np.float64)
+        out = np.zeros(6)
+        d = bunch.desc()
+        _check(load().sc3d_tail_fraction(self._ctx, C.byref(d),
+                                         c.ctypes.data_as(C.POINTER(C.c_double)),
+                                         w.ctypes.data_as(C.POINTER(C.c_double)), float(nsigma),
+                                         out.ctypes.data_as(C.POINTER(C.c_double)),
+                                         _stream_ptr(stream)))
+        return out
 
     def set_timing(self, enable=True):
         _check(load().sc3d_set_timing(self._ctx, int(enable)))

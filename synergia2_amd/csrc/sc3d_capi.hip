@@ -560,6 +560,32 @@ int sc3d_moments(sc3d_ctx* c, const sc3d_bunch* b, double mean[6], double mom2[3
     return SC3D_OK;
 }
 
+int sc3d_tail_fraction(sc3d_ctx* c, const sc3d_bunch* b, const double center[6],
+                       const double width[6], double nsigma, double frac[6], void* stream)
+{
+    if (!c || !b || !center || !width || !frac)
+        return fail(SC3D_ERR_ARG, "sc3d_tail_fraction: null argument");
+    const int64_t n = b->n_active;
+    if (n < 0 || b->stride < n || (n > 0 && (!b->parts || !b->masks)))
+        return fail(SC3D_ERR_ARG, "sc3d_tail_fraction: bad bunch");
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->tails)
+        if (int rc = dmalloc(c, &c->tails, 8)) return rc;
+    TailArgs a{};
+    for (int j = 0; j < 6; ++j) {
+        a.c[j] = center[j];
+        a.lim[j] = nsigma * width[j];
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    launch_tail_counts(b->parts, b->stride, b->masks, n, a, c->tails, st);
+    HIP_TRY(hipGetLastError());
+    unsigned long long h[6];
+    HIP_TRY(hipMemcpyAsync(h, c->tails, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int j = 0; j < 6; ++j) frac[j] = double(h[j]) / b->total_num;
+    return SC3D_OK;
+}
+
 int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                            const double mean[6], const double chol[21], uint64_t seed,
                            int64_t id0, void* stream)

@@ -46,6 +46,7 @@ struct sc3d_ctx {
     sc3d::MapProgram post{};  // maps fused into the kick kernel (post.n == 0: none)
     double* M = nullptr;      // moment partials (diagnostics)
     int64_t M_cap = 0;
+    unsigned long long* tails = nullptr;  // tail counts (diagnostics)
 };
 
 namespace sc3d {

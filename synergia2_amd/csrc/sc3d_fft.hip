@@ -44,6 +44,13 @@ template <int L> struct Lines {
     static constexpr int value = (1024 / L) > 8 ? (1024 / L) : 8;
 };
 
+// Column passes: enough lines per block that each strided load or store
+// moves a 128-byte run (16 FP32 or 8 FP64 complex values).
+template <int L, typename T2> struct ColLines {
+    static constexpr int runs = 128 / int(sizeof(T2));
+    static constexpr int value = Lines<L>::value > runs ? Lines<L>::value : runs;
+};
+
 template <int R, bool INV, typename T2>
 __device__ __forceinline__ void dft_small(T2* v)
 {
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ j
                                                    int Hx)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = Lines<L>::value;
+    constexpr int NL = ColLines<L, T2>::value;
     __shared__ T2 buf[NL * (L + 1)];
     const GridJob& J = jobs[blockIdx.y];
     T2* __restrict__ S = static_cast<T2*>(J.S);
@@ -334,11 +341,15 @@ __global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ j
 //       interior nodes; boundary nodes 0, as the reference leaves them)
 // ---------------------------------------------------------------------------
 
-// lines per block for K lines held per transform line, within 64 KiB of LDS
+// lines per block for K lines held per transform line: the column run
+// width, halved while the block would hold more than 80 KiB of LDS (two
+// blocks per CU on gfx950's 160 KiB)
 template <int L, int K, typename T2> struct FieldLines {
-    static constexpr int base = Lines<L>::value;
-    static constexpr int value =
-        (size_t(K) * base * (L + 1) * sizeof(T2) <= 65536) ? base : base / 2;
+    static constexpr int base = ColLines<L, T2>::value;
+    static constexpr size_t bytes(int nl) { return size_t(K) * nl * (L + 1) * sizeof(T2); }
+    static constexpr int value = bytes(base) <= 81920       ? base
+                                 : bytes(base / 2) <= 81920 ? base / 2
+                                                            : base / 4;
 };
 
 // v * (-i s) = (s v.y, -s v.x)
@@ -640,14 +651,18 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
 }
 
 // G2 / G3: real-even column transforms in place over the octant buffer.
-// Line l -> (plane = l / Hx, kx = l % Hx); element p of the doubled axis
-// reads the stored octant index min(p, L-p); writes real parts p <= L/2.
+// Real column c -> (plane = c / Hx, kx = c % Hx); element p of the doubled
+// axis reads the stored octant index min(p, L-p); writes real parts
+// p <= L/2.  The spectrum of a real even column is real, so two columns
+// share one complex transform: columns 2l and 2l+1 (adjacent kx) go in as
+// the real and imaginary parts of line l and come out as its real and
+// imaginary parts.
 // AXIS 1 (y): address (plane*(gy+1) + p)*Hx + kx, plane = Z <= gz.
 // AXIS 2 (z): address (p*(gy+1) + plane)*Hx + kx, plane = ky <= gy.
 template <int L, int AXIS>
 __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restrict__ jobs,
                                                          const double2* __restrict__ tw,
-                                                         int nlines, int gy, int Hx)
+                                                         int ncols, int gy, int Hx)
 {
     constexpr int NL = Lines<L>::value;
     __shared__ double2 buf[NL * (L + 1)];
@@ -656,25 +671,29 @@ __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restri
     const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
     const int l0 = blockIdx.x * NL;
-    auto addr = [&](int l, int p) -> int64_t {
-        int plane = l / Hx, kx = l - plane * Hx;
+    auto addr = [&](int c, int p) -> int64_t {
+        int plane = c / Hx, kx = c - plane * Hx;
         if (AXIS == 1) return (int64_t(plane) * (gy + 1) + p) * Hx + kx;
         return (int64_t(p) * (gy + 1) + plane) * Hx + kx;
     };
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        double2 v;
-        v.y = 0.0;
-        v.x = l < nlines ? ghat[addr(l, p <= L / 2 ? p : L - p)] : 0.0;
-        buf[line * (L + 1) + p] = v;
+    // element e: p = e / (2 NL), column q = e % (2 NL) of the block's 2 NL
+    // consecutive columns (runs of 2 NL doubles along kx)
+    for (int e = threadIdx.x; e < 2 * NL * L; e += kThreads) {
+        int p = e / (2 * NL), q = e % (2 * NL);
+        int c = 2 * l0 + q;
+        double v = c < ncols ? ghat[addr(c, p <= L / 2 ? p : L - p)] : 0.0;
+        double2* slot = &buf[(q >> 1) * (L + 1) + p];
+        if (q & 1) slot->y = v; else slot->x = v;
     }
     __syncthreads();
     lds_fft<double2, L, NL, false>(buf, tw);
-    for (int e = threadIdx.x; e < NL * (L / 2 + 1); e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        if (l < nlines) ghat[addr(l, p)] = buf[line * (L + 1) + p].x;
+    for (int e = threadIdx.x; e < 2 * NL * (L / 2 + 1); e += kThreads) {
+        int p = e / (2 * NL), q = e % (2 * NL);
+        int c = 2 * l0 + q;
+        if (c < ncols) {
+            const double2 z = buf[(q >> 1) * (L + 1) + p];
+            ghat[addr(c, p)] = (q & 1) ? z.y : z.x;
+        }
     }
 }
 
@@ -725,20 +744,20 @@ template <int L> struct GreenY {
     static void run(const Geometry& g, const GridJob* jobs, int nj, const double2* tw,
                     hipStream_t st)
     {
-        int lines = (g.gz + 1) * g.Hx;
-        hipLaunchKernelGGL((k_green_cols<L, 1>), dim3(nblocks(lines, Lines<L>::value), nj),
-                           dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
-                           g.Hx);
+        int cols = (g.gz + 1) * g.Hx;
+        hipLaunchKernelGGL((k_green_cols<L, 1>),
+                           dim3(nblocks(cols, 2 * Lines<L>::value), nj), dim3(kThreads), 0, st,
+                           jobs, tw + twiddle_offset(L), cols, g.gy, g.Hx);
     }
 };
 template <int L> struct GreenZ {
     static void run(const Geometry& g, const GridJob* jobs, int nj, const double2* tw,
                     hipStream_t st)
     {
-        int lines = (g.gy + 1) * g.Hx;
-        hipLaunchKernelGGL((k_green_cols<L, 2>), dim3(nblocks(lines, Lines<L>::value), nj),
-                           dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
-                           g.Hx);
+        int cols = (g.gy + 1) * g.Hx;
+        hipLaunchKernelGGL((k_green_cols<L, 2>),
+                           dim3(nblocks(cols, 2 * Lines<L>::value), nj), dim3(kThreads), 0, st,
+                           jobs, tw + twiddle_offset(L), cols, g.gy, g.Hx);
     }
 };
 
@@ -781,8 +800,9 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 0>), dim3(nblocks(lines, Lines<L>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
+            hipLaunchKernelGGL((k_cols<R, L, 0>),
+                               dim3(nblocks(lines, ColLines<L, T2>::value), nj), dim3(kThreads),
+                               0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
                                int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.gy, g.Dy, g.gy, g.Hx);
         }
     };
@@ -791,8 +811,9 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 1>), dim3(nblocks(lines, Lines<L>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
+            hipLaunchKernelGGL((k_cols<R, L, 1>),
+                               dim3(nblocks(lines, ColLines<L, T2>::value), nj), dim3(kThreads),
+                               0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
                                int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.Dy, g.gy, g.gy, g.Hx);
         }
     };
@@ -801,8 +822,9 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.Dy * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 2>), dim3(nblocks(lines, Lines<L>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, lines,
+            hipLaunchKernelGGL((k_cols<R, L, 2>),
+                               dim3(nblocks(lines, ColLines<L, T2>::value), nj), dim3(kThreads),
+                               0, st, jobs, tw + twiddle_offset(L), lines, lines,
                                int64_t(0), int64_t(lines), g.gz, g.gz, g.gy, g.Hx);
         }
     };

@@ -83,6 +83,12 @@ void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                  double m, const MapProgram& prog, const Sched& sc, hipStream_t st);
 void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                     double total_num, double* P, double* out, hipStream_t st);
+struct TailArgs {
+    double c[6];    // centre per plane
+    double lim[6];  // |p - c| beyond this counts as tail
+};
+void launch_tail_counts(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                        const TailArgs& a, unsigned long long* counts, hipStream_t st);
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                      const PopulateArgs& a, hipStream_t st);
 

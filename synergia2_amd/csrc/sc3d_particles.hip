@@ -143,6 +143,40 @@ __global__ __launch_bounds__(kThreads) void k_moments_finish(const double* __res
         for (int q = 0; q < K; ++q) out[q] = s[q] / total_num;
 }
 
+// ---------------------------------------------------------------------------
+// Tail counts (diagnostic of the halo comparison of long tracking): per
+// plane j, active particles with |p_j - c_j| > lim_j.  Integer counts, so
+// the result does not depend on the reduction order.
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n)
+{
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(kThreads) void k_tail_counts(const double* __restrict__ parts,
+                                                          int64_t stride,
+                                                          const uint8_t* __restrict__ masks,
+                                                          int64_t n, TailArgs a,
+                                                          unsigned long long* __restrict__ counts)
+{
+    unsigned int cnt[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kThreads) {
+        if (!masks[i]) continue;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) cnt[j] += fabs(parts[i + j * stride] - a.c[j]) > a.lim[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+#pragma unroll
+        for (int w = 1; w < 64; w *= 2) cnt[j] += __shfl_xor(cnt[j], w, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            if (cnt[j]) atomicAdd(&counts[j], (unsigned long long)cnt[j]);
+}
+
 void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                     double total_num, double* P, double* out /* 6 + 21 */, hipStream_t st)
 {
@@ -356,7 +390,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_count(
         if (valid) atomicAdd(&hist[(key >> shift) & (nb - 1)], 1);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < nb; e += kThreads) counts[int64_t(e) * nblk + blockIdx.x] = hist[e];
+    for (int e = threadIdx.x; e < nb; e += kThreads)
+        counts[int64_t(e) * nblk + blockIdx.x] = hist[e];
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts (in
@@ -897,6 +932,16 @@ static int grid_for(int64_t n, int cap = 8192)
     if (b < 1) b = 1;
     return int(b < cap ? b : cap);
 }
+
+void launch_tail_counts(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
+                        const TailArgs& a, unsigned long long* counts, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counts, 6);
+    if (n > 0)
+        hipLaunchKernelGGL(k_tail_counts, dim3(grid_for(n, 2048)), dim3(kThreads), 0, st, parts,
+                           stride, masks, n, a, counts);
+}
+
 
 void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                    double* P, const DomainFinishArgs& a, DomainRec* dom, const Sched& sc,

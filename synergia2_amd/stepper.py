@@ -50,6 +50,51 @@ def split_operator_steps(elements, num_steps):
     return steps
 
 
+def split_operator_steps_elements(elements, steps_per_element=1):
+    """Split_operator_stepper_elements::apply_impl
+    (simulation/split_operator_stepper_elements.cc:4-51): every element of
+    non-zero length gets steps_per_element equal steps, each = first half of
+    the step's slice of that element, the kick, second half.  Same triple
+    format as split_operator_steps."""
+    if steps_per_element < 1:
+        raise ValueError("Split_operator_stepper_elements: steps_per_element must be >= 1")
+    steps = []
+    for kind, length, k1 in elements:
+        if length == 0.0:
+            continue
+        step = length / steps_per_element
+        for i in range(steps_per_element):
+            steps.append(([(kind, 0.5 * step, k1)], [(kind, 0.5 * step, k1)], step))
+    return steps
+
+
+def halo_mix_parts(n, chol, seed, halo_fraction=0.1, halo_radius=3.0):
+    """(7, n) host particles: a 6D Gaussian (mean 0, packed lower factor
+    chol) of which the last halo_fraction of the slots are replaced by a
+    transverse tail population, uniform inside the 4D (x, x', y, y')
+    hyper-ellipsoid of radius halo_radius in the Gaussian's normalised
+    coordinates (a K-V-like uniform ellipse at halo_radius sigma), with
+    Gaussian (cdt, dp/p).  numpy Generator seeded with `seed`; id = slot."""
+    rng = np.random.default_rng(seed)
+    L = np.zeros((6, 6))
+    q = 0
+    for r in range(6):
+        for c in range(r + 1):
+            L[r, c] = chol[q]
+            q += 1
+    u = rng.standard_normal((6, n))
+    nh = int(round(halo_fraction * n))
+    if nh:
+        g = rng.standard_normal((4, nh))
+        g /= np.linalg.norm(g, axis=0)
+        rad = halo_radius * rng.uniform(0.0, 1.0, nh) ** 0.25  # uniform in the 4-ball
+        u[:4, n - nh:] = g * rad
+    p = np.zeros((7, n))
+    p[:6] = L @ u
+    p[6] = np.arange(n)
+    return p
+
+
 def cell_matrix(elements, plane):
     """Linear thick-lens transfer matrix of an element list (plane 0 = x,
     1 = y; a focusing k1 > 0 focuses x)."""
@@ -116,11 +161,11 @@ class Propagator:
     collective kick (Propagator::propagate / do_step,
     simulation/propagator.cc:36-118,157-283, without diagnostics)."""
 
-    def __init__(self, elements, num_steps, bunch, operator, stream=None):
+    def __init__(self, elements, num_steps, bunch, operator, stream=None, steps=None):
         self.bunch = bunch
         self.op = operator
         self.stream = stream
-        self.steps = split_operator_steps(elements, num_steps)
+        self.steps = steps if steps is not None else split_operator_steps(elements, num_steps)
         self.kick_dt = [st[2] / (bunch.beta * sc.C_LIGHT) for st in self.steps]
         p, m = bunch.p, bunch.mass
         self.first = [sc.make_slices(st[0], p, m) for st in self.steps]

@@ -97,3 +97,44 @@ def test_job_footprint_host_function():
     s_elems = gz * 2 * gy * (gx + 1)
     assert f[0] >= 2 * s_elems * 8  # two FP32 spectrum workspaces
     assert SV.Job(10000, (32, 32, 64), 20, 1e-3, 1e-3, 0.01, seed=1).footprint(sc.FP64) > 0
+
+
+def test_split_operator_steps_elements():
+    """Split_operator_stepper_elements(1) on the 32-cell FODO ring: one kick
+    per element, 128 per turn, each step the element's length with half
+    slices either side (split_operator_stepper_elements.cc:4-51)."""
+    ring = S.fodo_channel(32)
+    steps = S.split_operator_steps_elements(ring, 1)
+    assert len(steps) == 128
+    for (kind, length, k1), (first, second, step) in zip(ring, steps):
+        assert step == length
+        assert first == [(kind, 0.5 * length, k1)] and second == first
+    two = S.split_operator_steps_elements(ring[:4], 2)
+    assert len(two) == 8 and all(abs(s[2] - e[1] / 2) < 1e-15 for s, e in
+                                  zip(two, [x for x in ring[:4] for _ in range(2)]))
+    assert abs(sum(s[2] for s in steps) - sum(e[1] for e in ring)) < 1e-12
+    with pytest.raises(ValueError):
+        S.split_operator_steps_elements(ring, 0)
+
+
+def test_halo_mix_population():
+    """90 % Gaussian + 10 % uniform 4D hyper-ellipsoid at 3 sigma: the tail
+    slots lie inside radius 3 in normalised coordinates, fill it uniformly
+    (radius^4 uniform), and the core keeps unit normalised variance."""
+    chol = S.matched_chol_sigma(S.fodo_channel(1), 2e-3, 2e-3, 0.02 / 0.8418, 1e-4)
+    n = 200000
+    p = S.halo_mix_parts(n, chol, seed=3)
+    L = np.zeros((6, 6))
+    q = 0
+    for r in range(6):
+        for c in range(r + 1):
+            L[r, c] = chol[q]
+            q += 1
+    u = np.linalg.solve(L, p[:6])
+    nh = n // 10
+    rad = np.linalg.norm(u[:4, n - nh:], axis=0)
+    assert rad.max() <= 3.0 * (1 + 1e-12)
+    assert abs(np.mean((rad / 3.0) ** 4) - 0.5) < 0.01
+    core = u[:, :n - nh]
+    assert np.all(np.abs(core.std(axis=1) - 1) < 0.01)
+    assert np.array_equal(p[6], np.arange(n))
