@@ -33,6 +33,16 @@ namespace sc3d {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
+// LDS layout of the transform lines: line l, element p at
+// l * stride + p + p/8.  One pad slot every 8 elements spreads the radix-8
+// pass that writes with a stride of 8 elements over the banks; the odd
+// line stride spreads the column loads that write one element per line.
+template <int L> struct LdsLine {
+    static constexpr int stride = L + L / 8 + 1;
+    static __device__ __forceinline__ int pad(int p) { return p + (p >> 3); }
+    static __device__ __forceinline__ int at(int line, int p) { return line * stride + pad(p); }
+};
+
 bool fft_length_supported(int L)
 {
     return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
@@ -51,10 +61,68 @@ template <int L, typename T2> struct ColLines {
     static constexpr int value = Lines<L>::value > runs ? Lines<L>::value : runs;
 };
 
+// Lines per block, from a preferred count, halved (down to 2) while the
+// block's LDS (K lines per transform line plus the twiddle table) exceeds
+// the cap: the passes are latency-bound at low occupancy, and gfx950 has
+// 160 KiB of LDS per CU.
+// (Measured at 64x64x128 and 128x128x256: 32 KiB for FP64 lines, 48 KiB
+// for FP32 lines.)
+#ifndef SC3D_LDS_CAP_FP64
+#define SC3D_LDS_CAP_FP64 32768
+#endif
+#ifndef SC3D_LDS_CAP_FP32
+#define SC3D_LDS_CAP_FP32 49152
+#endif
+template <int L, int K, typename T2, int PREF> struct CappedLines {
+    static constexpr size_t cap = sizeof(T2) >= 16 ? SC3D_LDS_CAP_FP64 : SC3D_LDS_CAP_FP32;
+    static constexpr size_t bytes(int nl)
+    {
+        return (size_t(K) * nl * LdsLine<L>::stride + L) * sizeof(T2);
+    }
+    static constexpr int fit(int nl) { return (nl > 2 && bytes(nl) > cap) ? fit(nl / 2) : nl; }
+    static constexpr int value = fit(PREF);
+};
+// column passes holding K lines per transform line
+template <int L, int K, typename T2>
+using FieldLines = CappedLines<L, K, T2, ColLines<L, T2>::value>;
+// row passes holding K lines per row (rows are contiguous: no run width)
+template <int M, int K, typename T2>
+using RowLines = CappedLines<M, K, T2, Lines<M>::value>;
+// the Green's function column passes (FP64, two real columns per line)
+template <int L>
+using GreenLines = CappedLines<L, 1, double2, Lines<L>::value>;
+
 template <int R, bool INV, typename T2>
 __device__ __forceinline__ void dft_small(T2* v)
 {
-    if constexpr (R == 2) {
+    if constexpr (R == 8) {
+        // y_k = E_(k mod 4) + w8^k O_(k mod 4), E/O = DFT4 of the even/odd
+        // inputs, w8 = e^(-i pi/4) (inverse: conjugate), w8^(k+4) = -w8^k
+        T2 e[4] = {v[0], v[2], v[4], v[6]};
+        T2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft_small<4, INV>(e);
+        dft_small<4, INV>(o);
+        using F = decltype(v[0].x);
+        const F c = F(0.70710678118654752440);
+        T2 t1, t2, t3;
+        if (!INV) {
+            t1.x = (o[1].x + o[1].y) * c; t1.y = (o[1].y - o[1].x) * c;  // (1 - i) c
+            t2.x = o[2].y;                t2.y = -o[2].x;                // -i
+            t3.x = (o[3].y - o[3].x) * c; t3.y = -(o[3].x + o[3].y) * c; // (-1 - i) c
+        } else {
+            t1.x = (o[1].x - o[1].y) * c; t1.y = (o[1].x + o[1].y) * c;  // (1 + i) c
+            t2.x = -o[2].y;               t2.y = o[2].x;                 // i
+            t3.x = -(o[3].x + o[3].y) * c; t3.y = (o[3].x - o[3].y) * c; // (-1 + i) c
+        }
+        v[0].x = e[0].x + o[0].x; v[0].y = e[0].y + o[0].y;
+        v[4].x = e[0].x - o[0].x; v[4].y = e[0].y - o[0].y;
+        v[1].x = e[1].x + t1.x;   v[1].y = e[1].y + t1.y;
+        v[5].x = e[1].x - t1.x;   v[5].y = e[1].y - t1.y;
+        v[2].x = e[2].x + t2.x;   v[2].y = e[2].y + t2.y;
+        v[6].x = e[2].x - t2.x;   v[6].y = e[2].y - t2.y;
+        v[3].x = e[3].x + t3.x;   v[3].y = e[3].y + t3.y;
+        v[7].x = e[3].x - t3.x;   v[7].y = e[3].y - t3.y;
+    } else if constexpr (R == 2) {
         T2 a = v[0], b = v[1];
         v[0].x = a.x + b.x; v[0].y = a.y + b.y;
         v[1].x = a.x - b.x; v[1].y = a.y - b.y;
@@ -95,10 +163,10 @@ __device__ __forceinline__ void stockham_pass(T2* buf, const T2* __restrict__ tw
         int b = threadIdx.x + q * kThreads;
         if (TOT % kThreads == 0 || b < TOT) {
             int line = b / NB, j = b % NB;
-            const T2* lb = buf + line * (L + 1);
+            const T2* lb = buf + line * LdsLine<L>::stride;
             int k = j & (Ns - 1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = lb[j + r * NB];
+            for (int r = 0; r < R; ++r) v[q][r] = lb[LdsLine<L>::pad(j + r * NB)];
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 T2 w = tw[(r * k) << (LL - lns - LR)];
@@ -114,27 +182,59 @@ __device__ __forceinline__ void stockham_pass(T2* buf, const T2* __restrict__ tw
         int b = threadIdx.x + q * kThreads;
         if (TOT % kThreads == 0 || b < TOT) {
             int line = b / NB, j = b % NB;
-            T2* lb = buf + line * (L + 1);
+            T2* lb = buf + line * LdsLine<L>::stride;
             int k = j & (Ns - 1);
             int d = ((j >> lns) << (lns + LR)) + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lb[d + r * Ns] = v[q][r];
+            for (int r = 0; r < R; ++r) lb[LdsLine<L>::pad(d + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-// NL unnormalised length-L FFTs in LDS; forward sign -1 (INV: +1).  The
-// caller has synchronised after filling buf.
+// NL unnormalised length-L FFTs in LDS; forward sign -1 (INV: +1).  Radix-8
+// passes, then one radix-4 or radix-2 pass for the remaining factor.  tw is
+// the length-L twiddle table, normally the block's LDS copy (load_twiddles).
+// The caller has synchronised after filling buf.
 template <typename T2, int L, int NL, bool INV>
 __device__ __forceinline__ void lds_fft(T2* buf, const T2* __restrict__ tw)
 {
     constexpr int LOG = ilog2(L);
     int lns = 0;
 #pragma unroll
-    for (int p = 0; p < LOG / 2; ++p, lns += 2)
-        stockham_pass<T2, 4, L, NL, INV>(buf, tw, lns);
-    if constexpr (LOG & 1) stockham_pass<T2, 2, L, NL, INV>(buf, tw, lns);
+    for (int p = 0; p < LOG / 3; ++p, lns += 3)
+        stockham_pass<T2, 8, L, NL, INV>(buf, tw, lns);
+    if constexpr (LOG % 3 == 2) stockham_pass<T2, 4, L, NL, INV>(buf, tw, lns);
+    if constexpr (LOG % 3 == 1) stockham_pass<T2, 2, L, NL, INV>(buf, tw, lns);
+}
+
+// The block's LDS copy of a length-L twiddle table (filled before the
+// __syncthreads that precedes the first lds_fft).
+template <typename T2, int L>
+__device__ __forceinline__ void load_twiddles(T2* twl, const T2* __restrict__ tw)
+{
+    for (int e = threadIdx.x; e < L; e += kThreads) twl[e] = tw[e];
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx.x mod 8),
+// each with its own L2.  Column passes read runs of a few lines per block,
+// and neighbouring blocks share 128-byte lines, so the block index is
+// remapped to give every XCD a contiguous range of line groups.  The
+// launchers round the grid up to a multiple of 8 (the extra groups find no
+// lines and store nothing).
+#ifndef SC3D_XCD_REMAP
+#define SC3D_XCD_REMAP 1
+#endif
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_group()
+{
+#if SC3D_XCD_REMAP
+    const int per = int(gridDim.x) / kXcds;
+    return int(blockIdx.x % kXcds) * per + int(blockIdx.x / kXcds);
+#else
+    return int(blockIdx.x);
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +251,8 @@ __global__ __launch_bounds__(kThreads) void k_x_r2c(const GridJob* __restrict__ 
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = Lines<M>::value;
-    __shared__ T2 buf[NL * (M + 1)];
+    __shared__ T2 buf[NL * LdsLine<M>::stride];
+    __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
     const R* __restrict__ rho = static_cast<const R*>(J.rho);
     T2* __restrict__ S = static_cast<T2*>(J.S);
@@ -177,18 +278,19 @@ __global__ __launch_bounds__(kThreads) void k_x_r2c(const GridJob* __restrict__ 
         } else {
             v.x = R(0); v.y = R(0);
         }
-        buf[line * (M + 1) + n] = v;
+        buf[LdsLine<M>::at(line, n)] = v;
     }
+    load_twiddles<T2, M>(twl, twM);
     __syncthreads();
-    lds_fft<T2, M, NL, false>(buf, twM);
+    lds_fft<T2, M, NL, false>(buf, twl);
     // post-process: X[k] = 1/2 (A + (-i) w^k B), A = Z[k] + conj Z[M-k],
     // B = Z[k] - conj Z[M-k], w = e^{-2 pi i / 2M}; k = 0..M.
     for (int e = threadIdx.x; e < NL * Hx; e += kThreads) {
         int line = e / Hx, k = e % Hx;
         int r = r0 + line;
         if (r >= nrows) continue;
-        T2 zk = buf[line * (M + 1) + (k == M ? 0 : k)];
-        T2 zm = buf[line * (M + 1) + (k == 0 ? 0 : M - k)];
+        T2 zk = buf[LdsLine<M>::at(line, (k == M ? 0 : k))];
+        T2 zm = buf[LdsLine<M>::at(line, (k == 0 ? 0 : M - k))];
         T2 A, B;
         A.x = zk.x + zm.x; A.y = zk.y - zm.y;
         B.x = zk.x - zm.x; B.y = zk.y + zm.y;
@@ -215,7 +317,8 @@ __global__ __launch_bounds__(kThreads) void k_x_c2r(const GridJob* __restrict__ 
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = Lines<M>::value;
-    __shared__ T2 buf[NL * (M + 1)];
+    __shared__ T2 buf[NL * LdsLine<M>::stride];
+    __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
     const T2* __restrict__ S = static_cast<const T2*>(J.S);
     R* __restrict__ phi = static_cast<R*>(J.phi);
@@ -242,14 +345,15 @@ __global__ __launch_bounds__(kThreads) void k_x_c2r(const GridJob* __restrict__ 
         } else {
             v.x = R(0); v.y = R(0);
         }
-        buf[line * (M + 1) + k] = v;
+        buf[LdsLine<M>::at(line, k)] = v;
     }
+    load_twiddles<T2, M>(twl, twM);
     __syncthreads();
-    lds_fft<T2, M, NL, true>(buf, twM);
+    lds_fft<T2, M, NL, true>(buf, twl);
     T2* dst = reinterpret_cast<T2*>(phi + int64_t(r0) * M);
     for (int e = threadIdx.x; e < NL * (M / 2); e += kThreads) {
         int line = e / (M / 2), n = e % (M / 2);
-        if (r0 + line < nrows) dst[line * (M / 2) + n] = buf[line * (M + 1) + n];
+        if (r0 + line < nrows) dst[line * (M / 2) + n] = buf[LdsLine<M>::at(line, n)];
     }
 }
 
@@ -267,14 +371,15 @@ __global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ j
                                                    int Hx)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = ColLines<L, T2>::value;
-    __shared__ T2 buf[NL * (L + 1)];
+    constexpr int NL = FieldLines<L, 1, T2>::value;
+    __shared__ T2 buf[NL * LdsLine<L>::stride];
+    __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
     T2* __restrict__ S = static_cast<T2*>(J.S);
     const R* __restrict__ ghat = static_cast<const R*>(J.ghat);
     const DomainRec* dom = J.dom;
     if (dom->error) return;
-    const int l0 = blockIdx.x * NL;
+    const int l0 = xcd_group() * NL;
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
@@ -285,13 +390,14 @@ __global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ j
         } else {
             v.x = R(0); v.y = R(0);
         }
-        buf[line * (L + 1) + p] = v;
+        buf[LdsLine<L>::at(line, p)] = v;
     }
+    load_twiddles<T2, L>(twl, tw);
     __syncthreads();
     if constexpr (MODE == 1) {
-        lds_fft<T2, L, NL, true>(buf, tw);
+        lds_fft<T2, L, NL, true>(buf, twl);
     } else {
-        lds_fft<T2, L, NL, false>(buf, tw);
+        lds_fft<T2, L, NL, false>(buf, twl);
     }
     if constexpr (MODE == 2) {
         // line l = ky*Hx + kx over the full Dy rows; element p = kz.
@@ -304,20 +410,20 @@ __global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ j
             int my = ky <= Dy2 / 2 ? ky : Dy2 - ky;
             int mz = p <= L / 2 ? p : L - p;
             R gv = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
-            T2 v = buf[line * (L + 1) + p];
+            T2 v = buf[LdsLine<L>::at(line, p)];
             v.x = v.x * gv;
             v.y = v.y * gv;
-            buf[line * (L + 1) + p] = v;
+            buf[LdsLine<L>::at(line, p)] = v;
         }
         __syncthreads();
-        lds_fft<T2, L, NL, true>(buf, tw);
+        lds_fft<T2, L, NL, true>(buf, twl);
     }
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
         if (p < nstore && l < nlines) {
             int plane = l / lpp, c = l - plane * lpp;
-            S[plane * plane_stride + p * pstride + c] = buf[line * (L + 1) + p];
+            S[plane * plane_stride + p * pstride + c] = buf[LdsLine<L>::at(line, p)];
         }
     }
 }
@@ -341,16 +447,6 @@ __global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ j
 //       interior nodes; boundary nodes 0, as the reference leaves them)
 // ---------------------------------------------------------------------------
 
-// lines per block for K lines held per transform line: the column run
-// width, halved while the block would hold more than 80 KiB of LDS (two
-// blocks per CU on gfx950's 160 KiB)
-template <int L, int K, typename T2> struct FieldLines {
-    static constexpr int base = ColLines<L, T2>::value;
-    static constexpr size_t bytes(int nl) { return size_t(K) * nl * (L + 1) * sizeof(T2); }
-    static constexpr int value = bytes(base) <= 81920       ? base
-                                 : bytes(base / 2) <= 81920 ? base / 2
-                                                            : base / 4;
-};
 
 // v * (-i s) = (s v.y, -s v.x)
 template <typename T2, typename R>
@@ -376,7 +472,8 @@ __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict_
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = FieldLines<L, 2, T2>::value;
-    __shared__ T2 buf[2 * NL * (L + 1)];
+    __shared__ T2 buf[2 * NL * LdsLine<L>::stride];
+    __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
     T2* __restrict__ S = static_cast<T2*>(J.S);
     T2* __restrict__ S2 = static_cast<T2*>(J.S2);
@@ -384,7 +481,7 @@ __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict_
     const DomainRec* dom = J.dom;
     if (dom->error) return;
     const R ihz = R(dom->inv_h[2]);
-    const int l0 = blockIdx.x * NL;
+    const int l0 = xcd_group() * NL;
     const int Dy = nlines / Hx;
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
@@ -395,10 +492,11 @@ __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict_
         } else {
             v.x = R(0); v.y = R(0);
         }
-        buf[line * (L + 1) + p] = v;
+        buf[LdsLine<L>::at(line, p)] = v;
     }
+    load_twiddles<T2, L>(twl, tw);
     __syncthreads();
-    lds_fft<T2, L, NL, false>(buf, tw);
+    lds_fft<T2, L, NL, false>(buf, twl);
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
@@ -407,20 +505,20 @@ __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict_
         int my = ky <= Dy / 2 ? ky : Dy - ky;
         int mz = p <= L / 2 ? p : L - p;
         R gv = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
-        T2 v = buf[line * (L + 1) + p];
+        T2 v = buf[LdsLine<L>::at(line, p)];
         v.x = v.x * gv;
         v.y = v.y * gv;
-        buf[line * (L + 1) + p] = v;
-        buf[(NL + line) * (L + 1) + p] = mul_minus_i(v, sin_2pi(tw, p, L) * ihz);
+        buf[LdsLine<L>::at(line, p)] = v;
+        buf[LdsLine<L>::at((NL + line), p)] = mul_minus_i(v, sin_2pi(twl, p, L) * ihz);
     }
     __syncthreads();
-    lds_fft<T2, L, 2 * NL, true>(buf, tw);
+    lds_fft<T2, L, 2 * NL, true>(buf, twl);
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
         if (p < gz && l < nlines) {
-            S[int64_t(p) * nlines + l] = buf[line * (L + 1) + p];
-            S2[int64_t(p) * nlines + l] = buf[(NL + line) * (L + 1) + p];
+            S[int64_t(p) * nlines + l] = buf[LdsLine<L>::at(line, p)];
+            S2[int64_t(p) * nlines + l] = buf[LdsLine<L>::at((NL + line), p)];
         }
     }
 }
@@ -432,14 +530,15 @@ __global__ __launch_bounds__(kThreads) void k_y_field(const GridJob* __restrict_
 {
     using T2 = typename cplx<R>::type;
     constexpr int NL = FieldLines<L, 3, T2>::value;
-    __shared__ T2 buf[3 * NL * (L + 1)];
+    __shared__ T2 buf[3 * NL * LdsLine<L>::stride];
+    __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
     T2* __restrict__ S = static_cast<T2*>(J.S);
     T2* __restrict__ S2 = static_cast<T2*>(J.S2);
     const DomainRec* dom = J.dom;
     if (dom->error) return;
     const R ihy = R(dom->inv_h[1]);
-    const int l0 = blockIdx.x * NL;
+    const int l0 = xcd_group() * NL;
     const int64_t plane = int64_t(L) * Hx;  // L = Dy rows of Hx
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
@@ -454,21 +553,22 @@ __global__ __launch_bounds__(kThreads) void k_y_field(const GridJob* __restrict_
             v.x = R(0); v.y = R(0);
             vz = v;
         }
-        buf[line * (L + 1) + p] = v;
-        buf[(NL + line) * (L + 1) + p] = mul_minus_i(v, sin_2pi(tw, p, L) * ihy);
-        buf[(2 * NL + line) * (L + 1) + p] = vz;
+        buf[LdsLine<L>::at(line, p)] = v;
+        buf[LdsLine<L>::at((NL + line), p)] = mul_minus_i(v, sin_2pi(tw, p, L) * ihy);
+        buf[LdsLine<L>::at((2 * NL + line), p)] = vz;
     }
+    load_twiddles<T2, L>(twl, tw);
     __syncthreads();
-    lds_fft<T2, L, 3 * NL, true>(buf, tw);
+    lds_fft<T2, L, 3 * NL, true>(buf, twl);
     for (int e = threadIdx.x; e < NL * gy; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
         if (l < nlines) {
             int z = l / Hx, c = l - z * Hx;
             int64_t at = z * plane + int64_t(p) * Hx + c;
-            S[at] = buf[line * (L + 1) + p];
-            S[at + int64_t(gy) * Hx] = buf[(NL + line) * (L + 1) + p];
-            S2[at] = buf[(2 * NL + line) * (L + 1) + p];
+            S[at] = buf[LdsLine<L>::at(line, p)];
+            S[at + int64_t(gy) * Hx] = buf[LdsLine<L>::at((NL + line), p)];
+            S2[at] = buf[LdsLine<L>::at((2 * NL + line), p)];
         }
     }
 }
@@ -480,8 +580,9 @@ __global__ __launch_bounds__(kThreads) void k_x_field(
 {
     using T2 = typename cplx<R>::type;
     using T4 = typename vec4_of<R>::type;
-    constexpr int NL = FieldLines<M, 3, T2>::value;
-    __shared__ T2 buf[3 * NL * (M + 1)];
+    constexpr int NL = RowLines<M, 3, T2>::value;
+    __shared__ T2 buf[3 * NL * LdsLine<M>::stride];
+    __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
     const T2* __restrict__ S = static_cast<const T2*>(J.S);
     const T2* __restrict__ S2 = static_cast<const T2*>(J.S2);
@@ -518,11 +619,12 @@ __global__ __launch_bounds__(kThreads) void k_x_field(
             } else {
                 v.x = R(0); v.y = R(0);
             }
-            buf[(f * NL + line) * (M + 1) + k] = v;
+            buf[LdsLine<M>::at((f * NL + line), k)] = v;
         }
     }
+    load_twiddles<T2, M>(twl, twM);
     __syncthreads();
-    lds_fft<T2, M, 3 * NL, true>(buf, twM);
+    lds_fft<T2, M, 3 * NL, true>(buf, twl);
     for (int e = threadIdx.x; e < NL * M; e += kThreads) {
         int line = e / M, x = e % M;
         int r = r0 + line;
@@ -532,9 +634,9 @@ __global__ __launch_bounds__(kThreads) void k_x_field(
         o.x = R(0); o.y = R(0); o.z = R(0); o.w = R(0);
         if (x > 0 && x < M - 1 && y > 0 && y < gy - 1 && z > 0 && z < gz - 1) {
             const int n = x >> 1;
-            T2 a = buf[line * (M + 1) + n];
-            T2 b = buf[(NL + line) * (M + 1) + n];
-            T2 c = buf[(2 * NL + line) * (M + 1) + n];
+            T2 a = buf[LdsLine<M>::at(line, n)];
+            T2 b = buf[LdsLine<M>::at((NL + line), n)];
+            T2 c = buf[LdsLine<M>::at((2 * NL + line), n)];
             o.x = (x & 1) ? a.y : a.x;
             o.y = (x & 1) ? b.y : b.x;
             o.z = (x & 1) ? c.y : c.x;
@@ -590,7 +692,8 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
 {
     constexpr int NL = Lines<M>::value;
     constexpr int Dx = 2 * M;
-    __shared__ double2 buf[NL * (M + 1)];
+    __shared__ double2 buf[NL * LdsLine<M>::stride];
+    __shared__ double2 twl[M];
     __shared__ double row[NL * (M + 1)];
     const GridJob& J = jobs[blockIdx.y];
     double* __restrict__ ghat = J.ghat64;
@@ -622,6 +725,7 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
         }
         row[e] = v;
     }
+    load_twiddles<double2, M>(twl, twM);
     __syncthreads();
     for (int e = threadIdx.x; e < NL * M; e += kThreads) {
         int line = e / M, n = e % M;
@@ -631,17 +735,17 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
         double2 v;
         v.x = row[line * (M + 1) + m0];
         v.y = row[line * (M + 1) + m1];
-        buf[line * (M + 1) + n] = v;
+        buf[LdsLine<M>::at(line, n)] = v;
     }
     __syncthreads();
-    lds_fft<double2, M, NL, false>(buf, twM);
+    lds_fft<double2, M, NL, false>(buf, twl);
     const int Hx = M + 1;
     for (int e = threadIdx.x; e < NL * Hx; e += kThreads) {
         int line = e / Hx, k = e % Hx;
         int r = r0 + line;
         if (r >= nrows) continue;
-        double2 zk = buf[line * (M + 1) + (k == M ? 0 : k)];
-        double2 zm = buf[line * (M + 1) + (k == 0 ? 0 : M - k)];
+        double2 zk = buf[LdsLine<M>::at(line, (k == M ? 0 : k))];
+        double2 zm = buf[LdsLine<M>::at(line, (k == 0 ? 0 : M - k))];
         double2 A, B;
         A.x = zk.x + zm.x; A.y = zk.y - zm.y;
         B.x = zk.x - zm.x; B.y = zk.y + zm.y;
@@ -664,13 +768,14 @@ __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restri
                                                          const double2* __restrict__ tw,
                                                          int ncols, int gy, int Hx)
 {
-    constexpr int NL = Lines<L>::value;
-    __shared__ double2 buf[NL * (L + 1)];
+    constexpr int NL = GreenLines<L>::value;
+    __shared__ double2 buf[NL * LdsLine<L>::stride];
+    __shared__ double2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
     double* __restrict__ ghat = J.ghat64;
     const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
-    const int l0 = blockIdx.x * NL;
+    const int l0 = xcd_group() * NL;
     auto addr = [&](int c, int p) -> int64_t {
         int plane = c / Hx, kx = c - plane * Hx;
         if (AXIS == 1) return (int64_t(plane) * (gy + 1) + p) * Hx + kx;
@@ -682,16 +787,17 @@ __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restri
         int p = e / (2 * NL), q = e % (2 * NL);
         int c = 2 * l0 + q;
         double v = c < ncols ? ghat[addr(c, p <= L / 2 ? p : L - p)] : 0.0;
-        double2* slot = &buf[(q >> 1) * (L + 1) + p];
+        double2* slot = &buf[LdsLine<L>::at((q >> 1), p)];
         if (q & 1) slot->y = v; else slot->x = v;
     }
+    load_twiddles<double2, L>(twl, tw);
     __syncthreads();
-    lds_fft<double2, L, NL, false>(buf, tw);
+    lds_fft<double2, L, NL, false>(buf, twl);
     for (int e = threadIdx.x; e < 2 * NL * (L / 2 + 1); e += kThreads) {
         int p = e / (2 * NL), q = e % (2 * NL);
         int c = 2 * l0 + q;
         if (c < ncols) {
-            const double2 z = buf[(q >> 1) * (L + 1) + p];
+            const double2 z = buf[LdsLine<L>::at((q >> 1), p)];
             ghat[addr(c, p)] = (q & 1) ? z.y : z.x;
         }
     }
@@ -729,6 +835,11 @@ void dispatch_len(int L, A&&... a)
 }
 
 inline int nblocks(int64_t items, int per) { return int((items + per - 1) / per); }
+// grid of a column pass: a multiple of the XCD count (see xcd_group)
+inline int col_blocks(int64_t items, int per)
+{
+    return (nblocks(items, per) + kXcds - 1) / kXcds * kXcds;
+}
 
 template <int M> struct GreenX {
     static void run(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
@@ -746,7 +857,8 @@ template <int L> struct GreenY {
     {
         int cols = (g.gz + 1) * g.Hx;
         hipLaunchKernelGGL((k_green_cols<L, 1>),
-                           dim3(nblocks(cols, 2 * Lines<L>::value), nj), dim3(kThreads), 0, st,
+                           dim3(col_blocks(cols, 2 * GreenLines<L>::value), nj), dim3(kThreads), 0,
+                           st,
                            jobs, tw + twiddle_offset(L), cols, g.gy, g.Hx);
     }
 };
@@ -756,7 +868,8 @@ template <int L> struct GreenZ {
     {
         int cols = (g.gy + 1) * g.Hx;
         hipLaunchKernelGGL((k_green_cols<L, 2>),
-                           dim3(nblocks(cols, 2 * Lines<L>::value), nj), dim3(kThreads), 0, st,
+                           dim3(col_blocks(cols, 2 * GreenLines<L>::value), nj), dim3(kThreads), 0,
+                           st,
                            jobs, tw + twiddle_offset(L), cols, g.gy, g.Hx);
     }
 };
@@ -801,8 +914,8 @@ template <typename R> struct SolveX {
         {
             int lines = g.gz * g.Hx;
             hipLaunchKernelGGL((k_cols<R, L, 0>),
-                               dim3(nblocks(lines, ColLines<L, T2>::value), nj), dim3(kThreads),
-                               0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
+                               dim3(col_blocks(lines, FieldLines<L, 1, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
                                int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.gy, g.Dy, g.gy, g.Hx);
         }
     };
@@ -812,8 +925,8 @@ template <typename R> struct SolveX {
         {
             int lines = g.gz * g.Hx;
             hipLaunchKernelGGL((k_cols<R, L, 1>),
-                               dim3(nblocks(lines, ColLines<L, T2>::value), nj), dim3(kThreads),
-                               0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
+                               dim3(col_blocks(lines, FieldLines<L, 1, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
                                int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.Dy, g.gy, g.gy, g.Hx);
         }
     };
@@ -823,8 +936,8 @@ template <typename R> struct SolveX {
         {
             int lines = g.Dy * g.Hx;
             hipLaunchKernelGGL((k_cols<R, L, 2>),
-                               dim3(nblocks(lines, ColLines<L, T2>::value), nj), dim3(kThreads),
-                               0, st, jobs, tw + twiddle_offset(L), lines, lines,
+                               dim3(col_blocks(lines, FieldLines<L, 1, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, lines,
                                int64_t(0), int64_t(lines), g.gz, g.gz, g.gy, g.Hx);
         }
     };
@@ -834,7 +947,7 @@ template <typename R> struct SolveX {
         {
             int lines = g.Dy * g.Hx;
             hipLaunchKernelGGL((k_z_field<R, L>),
-                               dim3(nblocks(lines, FieldLines<L, 2, T2>::value), nj),
+                               dim3(col_blocks(lines, FieldLines<L, 2, T2>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gz,
                                g.gy, g.Hx);
         }
@@ -845,7 +958,7 @@ template <typename R> struct SolveX {
         {
             int lines = g.gz * g.Hx;
             hipLaunchKernelGGL((k_y_field<R, L>),
-                               dim3(nblocks(lines, FieldLines<L, 3, T2>::value), nj),
+                               dim3(col_blocks(lines, FieldLines<L, 3, T2>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
                                g.Hx);
         }
@@ -856,7 +969,7 @@ template <typename R> struct SolveX {
         {
             int rows = g.gz * g.gy;
             hipLaunchKernelGGL((k_x_field<R, M>),
-                               dim3(nblocks(rows, FieldLines<M, 3, T2>::value), nj),
+                               dim3(nblocks(rows, RowLines<M, 3, T2>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
                                tw + twiddle_offset(2 * M), g.gy, g.gz, g.Dy);
         }

@@ -732,11 +732,13 @@ __global__ __launch_bounds__(kThreads) void k_deposit_f32(const double* __restri
 }
 
 // Peak of the unit-weight FP32 density (all entries >= 0, so the IEEE bit
-// patterns order like the values): wavefront max, one integer atomicMax per
-// wavefront into the domain record (cleared by k_domain_finish each apply).
+// patterns order like the values): block max (wavefront butterflies, then
+// LDS), one integer atomicMax per block into the domain record (cleared by
+// k_domain_finish each apply); the grid is bounded so few atomics meet.
 __global__ __launch_bounds__(kThreads) void k_rho_peak(const float* __restrict__ rho, int64_t G,
                                                        DomainRec* __restrict__ dom)
 {
+    __shared__ float wmax[kThreads / 64];
     if (dom->error) return;
     float m = 0.0f;
     for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < G;
@@ -744,7 +746,13 @@ __global__ __launch_bounds__(kThreads) void k_rho_peak(const float* __restrict__
         m = fmaxf(m, rho[i]);
 #pragma unroll
     for (int w = 1; w < 64; w *= 2) m = fmaxf(m, __shfl_xor(m, w, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(&dom->rho_peak, __float_as_int(m));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = wmax[0];
+        for (int w = 1; w < kThreads / 64; ++w) b = fmaxf(b, wmax[w]);
+        if (b > 0.0f) atomicMax(&dom->rho_peak, __float_as_int(b));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1045,7 +1053,7 @@ void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* m
         hipLaunchKernelGGL(k_deposit_f32, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
                            stride, masks, perm, n, gx, gy, gz, dom, rho, ctl);
     const int64_t G = int64_t(gx) * gy * gz;
-    hipLaunchKernelGGL(k_rho_peak, dim3(grid_for(G, 1024)), dim3(kThreads), 0, st, rho, G, dom);
+    hipLaunchKernelGGL(k_rho_peak, dim3(grid_for(G / 8, 512)), dim3(kThreads), 0, st, rho, G, dom);
 }
 
 template <typename R>
