@@ -465,14 +465,31 @@ __device__ __forceinline__ auto sin_2pi(const T2* __restrict__ tw, int k, int L)
     return (k == 0 || 2 * k == L) ? decltype(tw[0].y)(0) : -tw[k].y;
 }
 
+// The z pass holds one LDS line per column: the Green-multiplied spectrum
+// stays in registers (each thread keeps the elements it multiplied) for the
+// second inverse transform, and the Green values are fetched before the
+// forward transform so that both global loads are in flight together.
+// Lines per block: at most SC3D_ZF_LINES (measured at 128x128x256: 4 lines
+// 287 us, 8 lines 323 us, 16 lines 424 us for the FP32 field passes; the
+// pass is latency-bound and more lines cost occupancy).
+#ifndef SC3D_ZF_LINES
+#define SC3D_ZF_LINES 4
+#endif
+template <int L, typename T2>
+struct ZFieldLines {
+    static constexpr int capped = FieldLines<L, 1, T2>::value;
+    static constexpr int value = capped < SC3D_ZF_LINES ? capped : SC3D_ZF_LINES;
+};
+
 template <typename R, int L>
 __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict__ jobs,
                                                       const typename cplx<R>::type* __restrict__ tw,
                                                       int nlines, int gz, int gy, int Hx)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = FieldLines<L, 2, T2>::value;
-    __shared__ T2 buf[2 * NL * LdsLine<L>::stride];
+    constexpr int NL = ZFieldLines<L, T2>::value;
+    constexpr int PER = (NL * L + kThreads - 1) / kThreads;
+    __shared__ T2 buf[NL * LdsLine<L>::stride];
     __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
     T2* __restrict__ S = static_cast<T2*>(J.S);
@@ -483,6 +500,21 @@ __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict_
     const R ihz = R(dom->inv_h[2]);
     const int l0 = xcd_group() * NL;
     const int Dy = nlines / Hx;
+    // Green values of the elements this thread multiplies (element e =
+    // threadIdx.x + q * kThreads: kz = e / NL, line = e % NL)
+    R gv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * kThreads;
+        const int p = e / NL, l = l0 + e % NL;
+        gv[q] = R(0);
+        if (e < NL * L && l < nlines) {
+            const int ky = l / Hx, kx = l - ky * Hx;
+            const int my = ky <= Dy / 2 ? ky : Dy - ky;
+            const int mz = p <= L / 2 ? p : L - p;
+            gv[q] = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
+        }
+    }
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
@@ -497,29 +529,41 @@ __global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict_
     load_twiddles<T2, L>(twl, tw);
     __syncthreads();
     lds_fft<T2, L, NL, false>(buf, twl);
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        if (l >= nlines) continue;
-        int ky = l / Hx, kx = l - ky * Hx;
-        int my = ky <= Dy / 2 ? ky : Dy - ky;
-        int mz = p <= L / 2 ? p : L - p;
-        R gv = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
-        T2 v = buf[LdsLine<L>::at(line, p)];
-        v.x = v.x * gv;
-        v.y = v.y * gv;
-        buf[LdsLine<L>::at(line, p)] = v;
-        buf[LdsLine<L>::at((NL + line), p)] = mul_minus_i(v, sin_2pi(twl, p, L) * ihz);
+    T2 keep[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * kThreads;
+        if (e < NL * L) {
+            const int p = e / NL, line = e % NL;
+            T2 v = buf[LdsLine<L>::at(line, p)];
+            v.x = v.x * gv[q];
+            v.y = v.y * gv[q];
+            buf[LdsLine<L>::at(line, p)] = v;
+            keep[q] = v;
+        }
     }
     __syncthreads();
-    lds_fft<T2, L, 2 * NL, true>(buf, twl);
+    lds_fft<T2, L, NL, true>(buf, twl);
     for (int e = threadIdx.x; e < NL * L; e += kThreads) {
         int p = e / NL, line = e % NL;
         int l = l0 + line;
-        if (p < gz && l < nlines) {
-            S[int64_t(p) * nlines + l] = buf[LdsLine<L>::at(line, p)];
-            S2[int64_t(p) * nlines + l] = buf[LdsLine<L>::at((NL + line), p)];
+        if (p < gz && l < nlines) S[int64_t(p) * nlines + l] = buf[LdsLine<L>::at(line, p)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * kThreads;
+        if (e < NL * L) {
+            const int p = e / NL, line = e % NL;
+            buf[LdsLine<L>::at(line, p)] = mul_minus_i(keep[q], sin_2pi(twl, p, L) * ihz);
         }
+    }
+    __syncthreads();
+    lds_fft<T2, L, NL, true>(buf, twl);
+    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
+        int p = e / NL, line = e % NL;
+        int l = l0 + line;
+        if (p < gz && l < nlines) S2[int64_t(p) * nlines + l] = buf[LdsLine<L>::at(line, p)];
     }
 }
 
@@ -947,7 +991,7 @@ template <typename R> struct SolveX {
         {
             int lines = g.Dy * g.Hx;
             hipLaunchKernelGGL((k_z_field<R, L>),
-                               dim3(col_blocks(lines, FieldLines<L, 2, T2>::value), nj),
+                               dim3(col_blocks(lines, ZFieldLines<L, T2>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gz,
                                g.gy, g.Hx);
         }

@@ -360,38 +360,112 @@ __device__ __forceinline__ uint64_t match_digit(int d, bool valid, int width)
     return peers;
 }
 
-// Histogram of one digit per tile (LDS atomics); KEYS: compute the cell keys
-// from the particles first (and store them), fusing the key pass into pass 0.
+#ifndef SC3D_XCD_PART
+#define SC3D_XCD_PART 1
+#endif
+
+// Block index of this workgroup in an XCD-contiguous order.  Workgroups of
+// a launch go to the 8 XCDs round-robin (workgroup b to XCD b % 8) and each
+// XCD has its own L2; renumbered so that XCD x runs the x-th eighth of the
+// blocks, neighbouring tiles (whose partial cache lines of the sort output,
+// the digit counts and the reorder buffers meet) share one L2.  Used by the
+// sort and reorder kernels (measured faster); the deposit and the kick keep
+// the dispatch order (measured slower with it).  Grids are rounded up to
+// multiples of 8 by the launchers; any other grid keeps the identity.
+__device__ __forceinline__ unsigned part_block()
+{
+#if SC3D_XCD_PART
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    if ((nb & 7u) == 0u) return (b & 7u) * (nb >> 3) + (b >> 3);
+#endif
+    return blockIdx.x;
+}
+
+// Exclusive prefix of one value per thread over the block (wavefront
+// shuffles, one LDS exchange of the wavefront sums).  `wsum` holds
+// kThreads/64 entries; callers separate two uses by a barrier.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* wsum, int32_t& total)
+{
+    constexpr int kWaves = kThreads / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const int32_t y = __shfl_up(x, s, 64);
+        if (lane >= s) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int32_t ws = wsum[w];
+        off += w < wave ? ws : 0;
+        tot += ws;
+    }
+    total = tot;
+    return off + x - v;
+}
+
+// Histogram of one digit per tile.  Every lane first loads (or, with KEYS,
+// computes from the particles and stores) its 16 keys, so all loads of the
+// tile are in flight together; each wavefront then counts its quarter of
+// the tile into its own LDS histogram, one update per distinct digit and
+// round (ballot match: after the first pass equal digits cluster), and the
+// four are summed.  KEYS fuses the key pass into pass 0.
 template <bool KEYS>
 __global__ __launch_bounds__(kThreads) void k_radix_count(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks, int gx,
     int gy, int gz, int32_t* __restrict__ keys, int64_t n, int shift, int width,
     int32_t* __restrict__ counts, int nblk, const DomainRec* __restrict__ dom)
 {
-    __shared__ int32_t hist[kMaxBuckets];
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kWaveItems = kSortTile / kWaves;
+    __shared__ int32_t hist[kWaves][kMaxBuckets];
     if (dom->error) return;
+    const unsigned blk = part_block();
+    if (blk >= unsigned(nblk)) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nb = 1 << width;
-    for (int e = threadIdx.x; e < nb; e += kThreads) hist[e] = 0;
-    __syncthreads();
-    const DomainRec d = *dom;
-    const int64_t base = int64_t(blockIdx.x) * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
-        const int64_t i = base + int64_t(r) * kThreads + threadIdx.x;
-        const bool valid = i < n;
-        int32_t key = 0;
-        if (valid) {
-            if (KEYS) {
-                key = cell_key(parts, stride, masks, i, gx, gy, gz, d);
-                keys[i] = key;
-            } else {
-                key = keys[i];
-            }
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) hist[w][t] = 0;
+    const int64_t base = int64_t(blk) * kSortTile + int64_t(wave) * kWaveItems;
+    int32_t kk[kSortItems];
+    if (KEYS) {
+        const DomainRec d = *dom;
+#pragma unroll
+        for (int r = 0; r < kSortItems; ++r) {
+            const int64_t i = base + r * 64 + lane;
+            kk[r] = i < n ? cell_key(parts, stride, masks, i, gx, gy, gz, d) : -1;
         }
-        if (valid) atomicAdd(&hist[(key >> shift) & (nb - 1)], 1);
+#pragma unroll
+        for (int r = 0; r < kSortItems; ++r) {
+            const int64_t i = base + r * 64 + lane;
+            if (i < n) keys[i] = kk[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kSortItems; ++r) {
+            const int64_t i = base + r * 64 + lane;
+            kk[r] = i < n ? keys[i] : -1;  // keys are >= 0
+        }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < nb; e += kThreads)
-        counts[int64_t(e) * nblk + blockIdx.x] = hist[e];
+    const uint64_t lt_mask = (1ull << lane) - 1;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const bool valid = kk[r] >= 0;
+        const int dg = (kk[r] >> shift) & (nb - 1);
+        const uint64_t peers = match_digit(dg, valid, width);
+        if (valid && __popcll(peers & lt_mask) == 0) hist[wave][dg] += __popcll(peers);
+    }
+    __syncthreads();
+    if (t < nb) {
+        int32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) c += hist[w][t];
+        counts[int64_t(t) * nblk + blk] = c;
+    }
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts (in
@@ -426,77 +500,106 @@ __global__ __launch_bounds__(kThreads) void k_radix_scan(int32_t* __restrict__ c
     if (threadIdx.x == kThreads - 1) totals[blockIdx.x] = part[kThreads - 1];
 }
 
-// Stable scatter.  Items are ranked in tile order: 16 rounds of 256; inside
-// a round, wavefront w holds items [64w, 64w+64); an item's rank among equal
-// digits of its wavefront comes from the ballot match; the lowest lane of
-// each digit group publishes the group size and later advances the running
-// count and clears its slot (no full clears per round).
+// Stable scatter of one tile (4096 items).  Wavefront w ranks the items
+// [1024w, 1024w + 1024) of the tile in 16 rounds of 64 with wave-private
+// digit counters (ballot match: rank among equal digits of the round, the
+// lowest lane of each digit group advances the counter), with no block
+// barrier inside the rounds.  The tile's digit counts come from the count
+// pass (difference of consecutive exclusive prefixes).  Every item is then
+// placed in LDS at its position in the tile's stable digit order, and the
+// tile is written out in that order, so each digit's items of the tile form
+// one contiguous run in the output (positions as a per-item scatter would
+// give them).
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int64_t n,
     int shift, int width, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals,
     int nblk, int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
     const DomainRec* dom)
 {
-    __shared__ int32_t base_d[kMaxBuckets];
-    __shared__ int32_t run_d[kMaxBuckets];
-    __shared__ int32_t wave_d[4][kMaxBuckets];
-    __shared__ int32_t chunk_sum[kThreads];
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kWaveItems = kSortTile / kWaves;
+    __shared__ int32_t base_d[kMaxBuckets];   // output position of the tile's first digit-d item
+    __shared__ int32_t start_d[kMaxBuckets];  // its position inside the tile
+    __shared__ int32_t wave_off[kWaves][kMaxBuckets];
+    __shared__ int32_t wsum[2][kWaves];
+    __shared__ int32_t lkey[kSortTile];
+    __shared__ int32_t lval[kSortTile];
     if (dom->error) return;
+    const unsigned blk = part_block();
+    if (blk >= unsigned(nblk)) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int nb = 1 << width;
-    const int per = nb / kThreads > 0 ? nb / kThreads : 1;  // nb is 256..2048 or smaller
-    // exclusive prefix of the digit totals (block scan over nb entries)
-    int32_t local = 0;
-    for (int k = 0; k < per; ++k) {
-        const int e = t * per + k;
-        if (e < nb) local += totals[e];
+    const int nb = 1 << width;  // <= kMaxBuckets == kThreads: thread t owns digit t
+    const int64_t tile = int64_t(blk) * kSortTile;
+    const int nvalid = int(n - tile < kSortTile ? n - tile : kSortTile);
+    // digit totals -> exclusive prefix over digits (global digit bases);
+    // this tile's digit counts -> local digit starts
+    const int32_t tot = t < nb ? totals[t] : 0;
+    int32_t cnt = 0, before = 0;
+    if (t < nb) {
+        before = counts[int64_t(t) * nblk + blk];
+        const int32_t next = blk + 1 < unsigned(nblk) ? counts[int64_t(t) * nblk + blk + 1] : tot;
+        cnt = next - before;
     }
-    chunk_sum[t] = local;
+    int32_t all_n, tile_n;
+    const int32_t digit_base = block_exclusive_scan(tot, wsum[0], all_n);
+    const int32_t local_start = block_exclusive_scan(cnt, wsum[1], tile_n);
+    if (t < nb) {
+        base_d[t] = digit_base + before;
+        start_d[t] = local_start;
+    }
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wave_off[w][t] = 0;
     __syncthreads();
-    for (int w = 1; w < kThreads; w *= 2) {
-        int32_t v = t >= w ? chunk_sum[t - w] : 0;
-        __syncthreads();
-        chunk_sum[t] += v;
-        __syncthreads();
-    }
-    int32_t run = chunk_sum[t] - local;
-    for (int k = 0; k < per; ++k) {
-        const int e = t * per + k;
-        if (e < nb) {
-            base_d[e] = run + counts[int64_t(e) * nblk + blockIdx.x];
-            run += totals[e];
-            run_d[e] = 0;
-            wave_d[0][e] = wave_d[1][e] = wave_d[2][e] = wave_d[3][e] = 0;
-        }
-    }
-    __syncthreads();
+    // ranking inside each wavefront's quarter of the tile
     const uint64_t lt_mask = (1ull << lane) - 1;
-    const int64_t tile = int64_t(blockIdx.x) * kSortTile;
+    int32_t kk[kSortItems], vv[kSortItems], rk[kSortItems];
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int64_t i = tile + int64_t(r) * kThreads + t;
+        const int64_t i = tile + int64_t(wave) * kWaveItems + r * 64 + lane;
         const bool valid = i < n;
         const int32_t key = valid ? keys_in[i] : 0;
         const int32_t val = valid ? (vals_in ? vals_in[i] : int32_t(i)) : 0;
         const int dg = (key >> shift) & (nb - 1);
         const uint64_t peers = match_digit(dg, valid, width);
         const int rank = __popcll(peers & lt_mask);
-        const bool leader = valid && rank == 0;
-        if (leader) wave_d[wave][dg] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int64_t pos = base_d[dg] + run_d[dg] + rank;
-            for (int w = 0; w < wave; ++w) pos += wave_d[w][dg];
-            if (pos >= 0 && pos < n) {  // always true for consistent counts
-                keys_out[pos] = key;
-                vals_out[pos] = val;
-            }
+        // every lane of a digit group reads the counter before its lowest
+        // lane advances it (LDS operations of a wavefront complete in order)
+        const int32_t seen = wave_off[wave][dg];
+        if (valid && rank == 0) wave_off[wave][dg] = seen + __popcll(peers);
+        kk[r] = valid ? key : -1;
+        vv[r] = val;
+        rk[r] = seen + rank;
+    }
+    __syncthreads();
+    if (t < nb) {  // exclusive prefix over wavefronts, per digit
+        int32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const int32_t c = wave_off[w][t];
+            wave_off[w][t] = acc;
+            acc += c;
         }
-        __syncthreads();
-        if (leader) {
-            atomicAdd(&run_d[dg], __popcll(peers));
-            wave_d[wave][dg] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        if (kk[r] < 0) continue;  // keys are >= 0; -1 marks lanes past n
+        const int dg = (kk[r] >> shift) & (nb - 1);
+        const int lpos = start_d[dg] + wave_off[wave][dg] + rk[r];
+        if (lpos >= 0 && lpos < kSortTile) {  // always true for consistent counts
+            lkey[lpos] = kk[r];
+            lval[lpos] = vv[r];
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    for (int j = t; j < nvalid; j += kThreads) {
+        const int32_t key = lkey[j];
+        const int dg = (key >> shift) & (nb - 1);
+        const int64_t pos = int64_t(base_d[dg]) + (j - start_d[dg]);
+        if (pos >= 0 && pos < n) {  // always true for consistent counts
+            keys_out[pos] = key;
+            vals_out[pos] = lval[j];
+        }
     }
 }
 
@@ -507,7 +610,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_particles(
     uint8_t* __restrict__ smask, const DomainRec* dom, const StepCtl* __restrict__ ctl)
 {
     if (dom->error || (ctl && !ctl->reorder)) return;
-    for (int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x; j < n;
+    for (int64_t j = int64_t(part_block()) * kThreads + threadIdx.x; j < n;
          j += int64_t(gridDim.x) * kThreads) {
         int64_t src = perm[j];
         if (src < 0 || src >= n) src = j;  // perm is a permutation of [0, n)
@@ -525,7 +628,7 @@ __global__ __launch_bounds__(kThreads) void k_copy_back(double* __restrict__ par
                                                         const StepCtl* __restrict__ ctl)
 {
     if (dom->error || (ctl && !ctl->reorder)) return;
-    for (int64_t j = blockIdx.x * int64_t(kThreads) + threadIdx.x; j < n;
+    for (int64_t j = int64_t(part_block()) * kThreads + threadIdx.x; j < n;
          j += int64_t(gridDim.x) * kThreads) {
 #pragma unroll
         for (int c = 0; c < 7; ++c) parts[j + c * stride] = scratch[j + c * n];
@@ -934,12 +1037,18 @@ __global__ __launch_bounds__(kThreads) void k_populate(double* __restrict__ part
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// Blocks for n items, at most cap, rounded up to a multiple of 8 (the
+// XCD-contiguous block order of part_block needs one; the kernels launched
+// on these grids let surplus blocks run empty).
 static int grid_for(int64_t n, int cap = 8192)
 {
     int64_t b = (n + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
-    return int(b < cap ? b : cap);
+    if (b > cap) b = cap;
+    return int((b + 7) / 8 * 8);
 }
+
+static unsigned round8(int64_t b) { return unsigned((b + 7) / 8 * 8); }
 
 void launch_tail_counts(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                         const TailArgs& a, unsigned long long* counts, hipStream_t st)
@@ -997,18 +1106,19 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
     for (int p = 0; p < passes; ++p) {
         const int shift = width * p;
         if (p == 0)
-            hipLaunchKernelGGL(k_radix_count<true>, dim3(nblk), dim3(kThreads), 0, st, parts,
-                               stride, masks, gx, gy, gz, b.keys0, n, shift, width, b.counts,
-                               nblk, dom);
+            hipLaunchKernelGGL(k_radix_count<true>, dim3(round8(nblk)), dim3(kThreads), 0, st,
+                               parts, stride, masks, gx, gy, gz, b.keys0, n, shift, width,
+                               b.counts, nblk, dom);
         else
-            hipLaunchKernelGGL(k_radix_count<false>, dim3(nblk), dim3(kThreads), 0, st,
+            hipLaunchKernelGGL(k_radix_count<false>, dim3(round8(nblk)), dim3(kThreads), 0, st,
                                (const double*)nullptr, int64_t(0), (const uint8_t*)nullptr, gx,
                                gy, gz, const_cast<int32_t*>(kin), n, shift, width, b.counts,
                                nblk, dom);
         hipLaunchKernelGGL(k_radix_scan, dim3(nb), dim3(kThreads), 0, st, b.counts, nblk,
                            b.totals, dom);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kThreads), 0, st, kin, vin, n, shift,
-                           width, b.counts, b.totals, nblk, kout[p & 1], vout[p & 1], dom);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(round8(nblk)), dim3(kThreads), 0, st, kin, vin,
+                           n, shift, width, b.counts, b.totals, nblk, kout[p & 1], vout[p & 1],
+                           dom);
         kin = kout[p & 1];
         vin = vout[p & 1];
     }
@@ -1039,7 +1149,7 @@ void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
 {
     int64_t nb = (n + kThreads - 1) / kThreads;
     if (nb < 1) return;
-    hipLaunchKernelGGL(k_deposit<R>, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts, stride,
+    hipLaunchKernelGGL(k_deposit<R>, dim3(round8(nb)), dim3(kThreads), 0, st, parts, stride,
                        masks, perm, n, gx, gy, gz, dom, rho, ctl);
 }
 
@@ -1050,7 +1160,7 @@ void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* m
 {
     int64_t nb = (n + kThreads - 1) / kThreads;
     if (nb >= 1)
-        hipLaunchKernelGGL(k_deposit_f32, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
+        hipLaunchKernelGGL(k_deposit_f32, dim3(round8(nb)), dim3(kThreads), 0, st, parts,
                            stride, masks, perm, n, gx, gy, gz, dom, rho, ctl);
     const int64_t G = int64_t(gx) * gy * gz;
     hipLaunchKernelGGL(k_rho_peak, dim3(grid_for(G / 8, 512)), dim3(kThreads), 0, st, rho, G, dom);
