@@ -64,8 +64,8 @@ void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, co
                     hipStream_t st);
 template <typename R>
 void launch_fill(R* p, int64_t n, hipStream_t st);
-// FP64: k_deposit.  FP32 (precision form): k_deposit_f32 with unit weights,
-// then k_rho_peak records the density peak in the domain record.
+// k_deposit<R, K> (FP32: unit weights, compensated run sums), then, in the
+// precision form, k_rho_peak records the density peak in the domain record.
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
                     const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,

@@ -165,6 +165,34 @@ def test_step_parity_variants(gpu, case):
     assert np.array_equal(mg[np.argsort(pg[6], kind="stable")], masks)
 
 
+@pytest.mark.parametrize("precision", [sc.FP64, sc.FP32_GRID])
+@pytest.mark.parametrize("dense", [True, False])
+def test_deposit_lane_widths(gpu, dense, precision):
+    """rho for both lane widths of the deposit (4 particles per lane on a
+    bunch with >= 6 particles per cell, else 2), walking the bunch right after
+    a reorder (sort_period 1) and walking the sort permutation (sort_period 0,
+    never reordered), with masked and off-grid particles and a count that is
+    no multiple of the lane width."""
+    shape = (16, 16, 32)  # G = 8192
+    n = 60001 if dense else 20001
+    parts = gaussian_parts(n, C1_SIGMA, seed=31 + int(dense))
+    rng = np.random.default_rng(8)
+    masks = (rng.random(n) > 0.05).astype(np.uint8)
+    parts[0, rng.choice(n, 50, replace=False)] += 1.0  # off the grid
+    fixed = ((0.0, 0.0, 0.0), (8e-3, 8e-3, 0.1))
+    dt = 0.1 / (BETA * sc.C_LIGHT)
+    print(f"case deposit dense={dense} precision={precision}", flush=True)
+    rc, out = oracle_run(oracle_kw(shape, fixed=fixed), parts, masks, 5e10, dt, want=("rho",))
+    assert rc == 0
+    tol = 1e-13 if precision == sc.FP64 else 2e-7
+    for sp in (1, 0):
+        op, _ = gpu_run(make_opts(shape, sort_period=sp, fixed=fixed, precision=precision), parts,
+                        masks, 5e10, dt)
+        err = rel_l2(op.debug(sc.DBG_RHO), out["rho"])
+        print("sort_period", sp, "rho rel-L2", err)
+        assert err <= tol
+
+
 def test_multi_step_parity(gpu):
     """Three kicks in a row (sort every 2nd apply) vs three oracle kicks."""
     n = 30000
