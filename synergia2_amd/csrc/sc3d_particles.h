@@ -57,23 +57,29 @@ void launch_step_begin(StepCtl* ctl, hipStream_t st);
 void launch_maps_sched(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                        const Sched& sc, hipStream_t st);
 int sort_passes(int64_t G);
+// ctl (served form): the sort kernels run only on the steps the device marks
+// for a reorder.
 void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
-                 int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st);
+                 int gy, int gz, const DomainRec* dom, SortBuffers& b, const StepCtl* ctl,
+                 hipStream_t st);
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
                     double* scratch, uint8_t* smask, const DomainRec* dom, const StepCtl* ctl,
                     hipStream_t st);
 template <typename R>
 void launch_fill(R* p, int64_t n, hipStream_t st);
-// k_deposit<R, K> (FP32: unit weights, compensated run sums), then, in the
-// precision form, k_rho_peak records the density peak in the domain record.
+// Deposit: local (k_deposit_local): windows of the bunch storage, sorted by
+// cell in LDS (the storage is cell-sorted at every reorder); else
+// (k_deposit): through the permutation `perm` of this step's sort.  FP32:
+// unit weights, compensated run sums; then k_rho_peak records the density
+// peak in the domain record.
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
-                    const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
-                    R* rho, const StepCtl* ctl, hipStream_t st);
+                    const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
+                    DomainRec* dom, R* rho, const StepCtl* ctl, hipStream_t st);
 template <>
 void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* masks,
-                           const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
-                           float* rho, const StepCtl* ctl, hipStream_t st);
+                           const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
+                           DomainRec* dom, float* rho, const StepCtl* ctl, hipStream_t st);
 template <typename R>
 void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
                    typename vec4<R>::type* E, hipStream_t st);

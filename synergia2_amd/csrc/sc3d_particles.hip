@@ -381,6 +381,16 @@ __device__ __forceinline__ unsigned part_block()
     return blockIdx.x;
 }
 
+// The per-tile digit counts: entry (digit d, tile b), digit-major (the scan
+// reads a digit's row contiguously; measured faster than tile-major rows,
+// whose column reads in the scan cost more than the count pass's scattered
+// stores save: sort phase 175 vs 183 us at 4e6 particles, 654 vs 683 us at
+// 1.6e7).
+__device__ __forceinline__ int64_t count_at(int d, int64_t b, int nblk)
+{
+    return int64_t(d) * nblk + b;
+}
+
 // Exclusive prefix of one value per thread over the block (wavefront
 // shuffles, one LDS exchange of the wavefront sums).  `wsum` holds
 // kThreads/64 entries; callers separate two uses by a barrier.
@@ -417,12 +427,13 @@ template <bool KEYS>
 __global__ __launch_bounds__(kThreads) void k_radix_count(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks, int gx,
     int gy, int gz, int32_t* __restrict__ keys, int64_t n, int shift, int width,
-    int32_t* __restrict__ counts, int nblk, const DomainRec* __restrict__ dom)
+    int32_t* __restrict__ counts, int nblk, const DomainRec* __restrict__ dom,
+    const StepCtl* __restrict__ ctl)
 {
     constexpr int kWaves = kThreads / 64;
     constexpr int kWaveItems = kSortTile / kWaves;
     __shared__ int32_t hist[kWaves][kMaxBuckets];
-    if (dom->error) return;
+    if (dom->error || (ctl && !ctl->reorder)) return;
     const unsigned blk = part_block();
     if (blk >= unsigned(nblk)) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -464,7 +475,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_count(
         int32_t c = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) c += hist[w][t];
-        counts[int64_t(t) * nblk + blk] = c;
+        counts[count_at(t, blk, nblk)] = c;
     }
 }
 
@@ -472,32 +483,27 @@ __global__ __launch_bounds__(kThreads) void k_radix_count(
 // place) and the digit total.
 __global__ __launch_bounds__(kThreads) void k_radix_scan(int32_t* __restrict__ counts, int nblk,
                                                          int32_t* __restrict__ totals,
-                                                         const DomainRec* dom)
+                                                         const DomainRec* dom,
+                                                         const StepCtl* __restrict__ ctl)
 {
-    __shared__ int32_t part[kThreads];
-    if (dom->error) return;
-    int32_t* c = counts + int64_t(blockIdx.x) * nblk;
+    __shared__ int32_t wsum[kThreads / 64];
+    if (dom->error || (ctl && !ctl->reorder)) return;
+    const int d = blockIdx.x;
     const int per = (nblk + kThreads - 1) / kThreads;
     const int b0 = threadIdx.x * per;
     int32_t s = 0;
     for (int k = 0; k < per; ++k)
-        if (b0 + k < nblk) s += c[b0 + k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 1; w < kThreads; w *= 2) {  // inclusive Hillis-Steele scan
-        int32_t v = threadIdx.x >= w ? part[threadIdx.x - w] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int32_t run = part[threadIdx.x] - s;
+        if (b0 + k < nblk) s += counts[count_at(d, b0 + k, nblk)];
+    int32_t total;
+    int32_t run = block_exclusive_scan(s, wsum, total);
     for (int k = 0; k < per; ++k)
         if (b0 + k < nblk) {
-            int32_t v = c[b0 + k];
-            c[b0 + k] = run;
+            const int64_t at = count_at(d, b0 + k, nblk);
+            const int32_t v = counts[at];
+            counts[at] = run;
             run += v;
         }
-    if (threadIdx.x == kThreads - 1) totals[blockIdx.x] = part[kThreads - 1];
+    if (threadIdx.x == 0) totals[d] = total;
 }
 
 // Stable scatter of one tile (4096 items).  Wavefront w ranks the items
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int64_t n,
     int shift, int width, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals,
     int nblk, int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
-    const DomainRec* dom)
+    const DomainRec* dom, const StepCtl* __restrict__ ctl)
 {
     constexpr int kWaves = kThreads / 64;
     constexpr int kWaveItems = kSortTile / kWaves;
@@ -524,7 +530,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     __shared__ int32_t wsum[2][kWaves];
     __shared__ int32_t lkey[kSortTile];
     __shared__ int32_t lval[kSortTile];
-    if (dom->error) return;
+    if (dom->error || (ctl && !ctl->reorder)) return;
     const unsigned blk = part_block();
     if (blk >= unsigned(nblk)) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -536,8 +542,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const int32_t tot = t < nb ? totals[t] : 0;
     int32_t cnt = 0, before = 0;
     if (t < nb) {
-        before = counts[int64_t(t) * nblk + blk];
-        const int32_t next = blk + 1 < unsigned(nblk) ? counts[int64_t(t) * nblk + blk + 1] : tot;
+        before = counts[count_at(t, blk, nblk)];
+        const int32_t next = blk + 1 < unsigned(nblk) ? counts[count_at(t, blk + 1, nblk)] : tot;
         cnt = next - before;
     }
     int32_t all_n, tile_n;
@@ -774,6 +780,71 @@ __device__ __forceinline__ void add_run(R* __restrict__ rho, int key, const Corn
     }
 }
 
+// The runs of one lane's K consecutive particles of a cell-sorted order and
+// the wavefront's combination (see above).  key[q] < 0 marks particles that
+// add nothing; those keys must differ from each other and from the keys of
+// every other lane of the wavefront.
+template <typename R, int K>
+__device__ __forceinline__ void deposit_runs(const int (&key)[K], const double (&ox)[K],
+                                             const double (&oy)[K], const double (&oz)[K],
+                                             double w0, R* __restrict__ rho, int gx, int gy,
+                                             int gz)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
+    // runs inside the lane: the leading run (head), the trailing run (cur at
+    // the end); runs in between are added at once
+    CornerSums<R> cur, head;
+    int cur_key = 0, head_key = 0, cur_start = 0;
+    bool single = true;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const CornerSums<R> p = corner_weights<R>(key[q] >= 0, ox[q], oy[q], oz[q], w0);
+        if (q == 0) {
+            cur = p;
+            cur_key = key[q];
+        } else if (key[q] == cur_key) {
+            cur.add(p);
+        } else {
+            if (cur_start == 0) {
+                head = cur;
+                head_key = cur_key;
+                single = false;
+            } else if (cur_key >= 0) {
+                add_run<R>(rho, cur_key, cur, gx, gy, gz);
+            }
+            cur = p;
+            cur_key = key[q];
+            cur_start = q;
+        }
+    }
+    if (single) head_key = cur_key;
+    // trailing runs across lanes: segmented inclusive scan (every lane takes
+    // part in every shuffle: a lane shuffle must never run under a divergent
+    // exec mask)
+    const int prev_tail = __shfl_up(cur_key, 1, 64);
+    const bool start = lane == 0 || !single || prev_tail != cur_key;
+    const int seg = __popcll(__ballot(start) & le_mask);
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const int seg_up = __shfl_up(seg, s, 64);
+        const CornerSums<R> up = cur.shfl_up(s);
+        if ((lane >= s) & (seg_up == seg)) cur.add(up);
+    }
+    // the total of the previous lane's trailing run continues this lane's
+    // leading run when the keys agree
+    const CornerSums<R> carry = cur.shfl_up(1);
+    const int next_head = __shfl_down(head_key, 1, 64);
+    if (!single && head_key >= 0) {
+        if (lane > 0 && prev_tail == head_key) head.add(carry);
+        add_run<R>(rho, head_key, head, gx, gy, gz);
+    }
+    if (cur_key >= 0 && (lane == 63 || next_head != cur_key))
+        add_run<R>(rho, cur_key, cur, gx, gy, gz);
+}
+
+// Deposit through the sort permutation of this step (every apply sorts;
+// used when the bunch storage is never reordered, sort_period 0).
 template <typename R, int K>
 __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__ parts,
                                                       int64_t stride,
@@ -789,9 +860,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
     const int32_t* __restrict__ perm = (ctl && ctl->reorder) ? nullptr : perm_in;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
-    const double w0 = dom->w0;
     const int lane = threadIdx.x & 63;
-    const uint64_t le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
     // this lane's K consecutive positions of the cell-sorted order
     const int64_t j0 = (blockIdx.x * int64_t(kThreads) + threadIdx.x) * K;
     int64_t idx[K];
@@ -826,65 +895,198 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
             Z[q] = parts[idx[q] + 4 * stride];
         }
     }
-    // runs inside the lane: the leading run (head), the trailing run (cur at
-    // the end); runs in between are added at once
-    CornerSums<R> cur, head;
-    int cur_key = 0, head_key = 0, cur_start = 0;
-    bool single = true;
+    int key[K];
+    double ox[K], oy[K], oz[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-        double ox = 0.0, oy = 0.0, oz = 0.0;
-        bool ok = false;
-        int key = -1 - (lane * K + q);  // off-grid particles: singletons
+        ox[q] = oy[q] = oz[q] = 0.0;
+        key[q] = -1 - (lane * K + q);  // off-grid particles: singletons
         if (M[q]) {
-            const int ix = cell_index(X[q], lx, ihx, ox);
-            const int iy = cell_index(Y[q], ly, ihy, oy);
-            const int iz = cell_index(Z[q], lz, ihz, oz);
-            ok = ix >= 0 && ix <= gx - 2 && iy >= 0 && iy <= gy - 2 && iz >= 0 && iz <= gz - 2;
-            if (ok) key = (iz * gy + iy) * gx + ix;
-        }
-        const CornerSums<R> p = corner_weights<R>(ok, ox, oy, oz, w0);
-        if (q == 0) {
-            cur = p;
-            cur_key = key;
-        } else if (key == cur_key) {
-            cur.add(p);
-        } else {
-            if (cur_start == 0) {
-                head = cur;
-                head_key = cur_key;
-                single = false;
-            } else if (cur_key >= 0) {
-                add_run<R>(rho, cur_key, cur, gx, gy, gz);
-            }
-            cur = p;
-            cur_key = key;
-            cur_start = q;
+            const int ix = cell_index(X[q], lx, ihx, ox[q]);
+            const int iy = cell_index(Y[q], ly, ihy, oy[q]);
+            const int iz = cell_index(Z[q], lz, ihz, oz[q]);
+            if (ix >= 0 && ix <= gx - 2 && iy >= 0 && iy <= gy - 2 && iz >= 0 && iz <= gz - 2)
+                key[q] = (iz * gy + iy) * gx + ix;
         }
     }
-    if (single) head_key = cur_key;
-    // trailing runs across lanes: segmented inclusive scan (every lane takes
-    // part in every shuffle: a lane shuffle must never run under a divergent
-    // exec mask)
-    const int prev_tail = __shfl_up(cur_key, 1, 64);
-    const bool start = lane == 0 || !single || prev_tail != cur_key;
-    const int seg = __popcll(__ballot(start) & le_mask);
+    deposit_runs<R, K>(key, ox, oy, oz, dom->w0, rho, gx, gy, gz);
+}
+
+// Deposit in windows of the bunch storage.  The storage is cell-sorted at
+// every reorder (sort_period applies) and particles move a few cells per
+// step, so a window of kWindow consecutive slots stays a compact group of
+// cells: the block loads its window coalesced, sorts (key, slot) by cell key
+// in LDS (stable LSD radix over the bits of key - min key of the window,
+// <= 8-bit digits, wave-private ranking as in the global scatter; skipped
+// when the window is already in cell order), and deposits it in that order
+// with the lane runs (4 particles per lane).  A cell whose particles fall in
+// several windows is added once per window.  Measured at 4e6 / 1.6e7
+// particles (deposit phase, us, sort_period 8): windows of 1024 with the
+// offsets held in LDS 158 / 590; 1024 re-reading the coordinates 141 / 493;
+// 2048 re-reading 137 / 530; 4096 re-reading 166 / 726.
+constexpr int kWindow = 1024;
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_deposit_local(const double* __restrict__ parts,
+                                                            int64_t stride,
+                                                            const uint8_t* __restrict__ masks,
+                                                            int64_t n, int gx, int gy, int gz,
+                                                            const DomainRec* __restrict__ dom,
+                                                            R* __restrict__ rho)
+{
+    constexpr int W = kWindow, PER = kWindow / kThreads, kWaves = kThreads / 64;
+    __shared__ int32_t skey[2][W];
+    __shared__ int32_t sslot[2][W];
+    __shared__ int32_t wave_off[kWaves][kMaxBuckets];
+    __shared__ int32_t start_d[kMaxBuckets];
+    __shared__ int32_t wsum[kWaves];
+    if (dom->error) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
+    const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
+    // off-grid, masked and absent slots sort last under G-1, which no cell
+    // on the grid has (ix <= gx-2)
+    const int sentinel = gx * gy * gz - 1;
+    const int64_t base = int64_t(blockIdx.x) * W;
 #pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const int seg_up = __shfl_up(seg, s, 64);
-        const CornerSums<R> up = cur.shfl_up(s);
-        if ((lane >= s) & (seg_up == seg)) cur.add(up);
+    for (int q = 0; q < PER; ++q) {
+        const int sl = q * kThreads + t;
+        const int64_t j = base + sl;
+        double x = 0.0, y = 0.0, z = 0.0;
+        bool m = false;
+        if (j < n) {
+            m = masks[j] != 0;
+            x = parts[j];
+            y = parts[j + 2 * stride];
+            z = parts[j + 4 * stride];
+        }
+        double ox, oy, oz;
+        int key = sentinel;
+        if (m) {
+            const int ix = cell_index(x, lx, ihx, ox);
+            const int iy = cell_index(y, ly, ihy, oy);
+            const int iz = cell_index(z, lz, ihz, oz);
+            if (ix >= 0 && ix <= gx - 2 && iy >= 0 && iy <= gy - 2 && iz >= 0 && iz <= gz - 2)
+                key = (iz * gy + iy) * gx + ix;
+        }
+        skey[0][sl] = key;
+        sslot[0][sl] = sl;
     }
-    // the total of the previous lane's trailing run continues this lane's
-    // leading run when the keys agree
-    const CornerSums<R> carry = cur.shfl_up(1);
-    const int next_head = __shfl_down(head_key, 1, 64);
-    if (!single && head_key >= 0) {
-        if (lane > 0 && prev_tail == head_key) head.add(carry);
-        add_run<R>(rho, head_key, head, gx, gy, gz);
+    // key range of the window (the sentinel excluded); a window already in
+    // cell order (right after a reorder, or where nothing crossed a cell
+    // boundary) is not sorted again
+    int kmin = sentinel, kmax = 0;
+    bool desc = false;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int sl = q * kThreads + t;
+        const int k = skey[0][sl];
+        if (k != sentinel) {
+            kmin = min(kmin, k);
+            kmax = max(kmax, k);
+        }
+        if (sl > 0 && skey[0][sl - 1] > k) desc = true;
     }
-    if (cur_key >= 0 && (lane == 63 || next_head != cur_key))
-        add_run<R>(rho, cur_key, cur, gx, gy, gz);
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2) {
+        kmin = min(kmin, __shfl_xor(kmin, w, 64));
+        kmax = max(kmax, __shfl_xor(kmax, w, 64));
+    }
+    if (lane == 0) {
+        wave_off[0][wave] = kmin;
+        wave_off[1][wave] = kmax;
+    }
+    desc = __syncthreads_or(desc);
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        kmin = min(kmin, wave_off[0][w]);
+        kmax = max(kmax, wave_off[1][w]);
+    }
+    // stable LSD radix sort of the window by key - kmin (the sentinel last)
+    int span_bits = 0;
+    if (desc) {
+        const int top = kmax >= kmin ? kmax - kmin + 1 : 1;
+        while ((1 << span_bits) <= top) ++span_bits;
+    }
+    const int passes = (span_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const int width = passes > 0 ? (span_bits + passes - 1) / passes : 1;
+    const int nb = 1 << width;
+    const uint64_t lt_mask = (1ull << lane) - 1;
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * width;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) wave_off[w][t] = 0;
+        __syncthreads();
+        int32_t kk[PER], ss[PER], rk[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int pos = wave * (PER * 64) + r * 64 + lane;
+            const int32_t key = skey[cur][pos];
+            const int rel = key != sentinel ? key - kmin : kmax - kmin + 1;
+            const int dg = (rel >> shift) & (nb - 1);
+            const uint64_t peers = match_digit(dg, true, width);
+            const int rank = __popcll(peers & lt_mask);
+            // every lane of a digit group reads the counter before its lowest
+            // lane advances it (LDS operations of a wavefront complete in order)
+            const int32_t seen = wave_off[wave][dg];
+            if (rank == 0) wave_off[wave][dg] = seen + __popcll(peers);
+            kk[r] = key;
+            ss[r] = sslot[cur][pos];
+            rk[r] = seen + rank;
+        }
+        __syncthreads();
+        int32_t tot = 0;
+        if (t < nb) {  // exclusive prefix over wavefronts, per digit
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const int32_t c = wave_off[w][t];
+                wave_off[w][t] = tot;
+                tot += c;
+            }
+        }
+        int32_t all;
+        const int32_t st0 = block_exclusive_scan(tot, wsum, all);
+        if (t < nb) start_d[t] = st0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int rel = kk[r] != sentinel ? kk[r] - kmin : kmax - kmin + 1;
+            const int dg = (rel >> shift) & (nb - 1);
+            const int pos = start_d[dg] + wave_off[wave][dg] + rk[r];
+            if (pos >= 0 && pos < W) {  // always true: the counts cover the window
+                skey[cur ^ 1][pos] = kk[r];
+                sslot[cur ^ 1][pos] = ss[r];
+            }
+        }
+        cur ^= 1;
+    }
+    __syncthreads();
+    // in chunks of 4 * kThreads sorted positions: lane t takes [4t, 4t + 4)
+#pragma unroll
+    for (int h = 0; h < PER / 4; ++h) {
+        int key[4];
+        double ox[4], oy[4], oz[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int pos = h * 4 * kThreads + 4 * t + q;
+            const int k = skey[cur][pos];
+            const int sl = sslot[cur][pos] & (W - 1);
+            key[q] = k != sentinel ? k : -1 - (lane * 4 + q);
+            // the window's coordinates again, in sorted order (cache hits;
+            // cheaper than holding the offsets in LDS, which costs occupancy)
+            ox[q] = oy[q] = oz[q] = 0.0;
+            if (k != sentinel) {
+                const int64_t j = base + sl;
+                cell_index(parts[j], lx, ihx, ox[q]);
+                cell_index(parts[j + 2 * stride], ly, ihy, oy[q]);
+                cell_index(parts[j + 4 * stride], lz, ihz, oz[q]);
+            }
+        }
+        deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, rho, gx, gy, gz);
+    }
 }
 
 // Peak of the unit-weight FP32 density (all entries >= 0, so the IEEE bit
@@ -1145,7 +1347,8 @@ static int sort_bits(int64_t G)
 }
 
 void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
-                 int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st)
+                 int gy, int gz, const DomainRec* dom, SortBuffers& b, const StepCtl* ctl,
+                 hipStream_t st)
 {
     const int nblk = int((n + kSortTile - 1) / kSortTile);
     const int bits = sort_bits(int64_t(gx) * gy * gz);
@@ -1161,17 +1364,17 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
         if (p == 0)
             hipLaunchKernelGGL(k_radix_count<true>, dim3(round8(nblk)), dim3(kThreads), 0, st,
                                parts, stride, masks, gx, gy, gz, b.keys0, n, shift, width,
-                               b.counts, nblk, dom);
+                               b.counts, nblk, dom, ctl);
         else
             hipLaunchKernelGGL(k_radix_count<false>, dim3(round8(nblk)), dim3(kThreads), 0, st,
                                (const double*)nullptr, int64_t(0), (const uint8_t*)nullptr, gx,
                                gy, gz, const_cast<int32_t*>(kin), n, shift, width, b.counts,
-                               nblk, dom);
+                               nblk, dom, ctl);
         hipLaunchKernelGGL(k_radix_scan, dim3(nb), dim3(kThreads), 0, st, b.counts, nblk,
-                           b.totals, dom);
+                           b.totals, dom, ctl);
         hipLaunchKernelGGL(k_radix_scatter, dim3(round8(nblk)), dim3(kThreads), 0, st, kin, vin,
                            n, shift, width, b.counts, b.totals, nblk, kout[p & 1], vout[p & 1],
-                           dom);
+                           dom, ctl);
         kin = kout[p & 1];
         vin = vout[p & 1];
     }
@@ -1219,19 +1422,36 @@ static void launch_deposit_k(const double* parts, int64_t stride, const uint8_t*
 }
 
 template <typename R>
-void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
-                    const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
-                    R* rho, const StepCtl* ctl, hipStream_t st)
+static void launch_deposit_local(const double* parts, int64_t stride, const uint8_t* masks,
+                                 int64_t n, int gx, int gy, int gz, DomainRec* dom, R* rho,
+                                 hipStream_t st)
 {
-    launch_deposit_k<R>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, ctl, st);
+    const int64_t nb = (n + kWindow - 1) / kWindow;
+    if (nb < 1) return;
+    hipLaunchKernelGGL(k_deposit_local<R>, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
+                       stride, masks, n, gx, gy, gz, dom, rho);
+}
+
+template <typename R>
+void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
+                    const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
+                    DomainRec* dom, R* rho, const StepCtl* ctl, hipStream_t st)
+{
+    if (local)
+        launch_deposit_local<R>(parts, stride, masks, n, gx, gy, gz, dom, rho, st);
+    else
+        launch_deposit_k<R>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, ctl, st);
 }
 
 template <>
 void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* masks,
-                           const int32_t* perm, int64_t n, int gx, int gy, int gz, DomainRec* dom,
-                           float* rho, const StepCtl* ctl, hipStream_t st)
+                           const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
+                           DomainRec* dom, float* rho, const StepCtl* ctl, hipStream_t st)
 {
-    launch_deposit_k<float>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, ctl, st);
+    if (local)
+        launch_deposit_local<float>(parts, stride, masks, n, gx, gy, gz, dom, rho, st);
+    else
+        launch_deposit_k<float>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, ctl, st);
     const int64_t G = int64_t(gx) * gy * gz;
     hipLaunchKernelGGL(k_rho_peak, dim3(grid_for(G / 8, 512)), dim3(kThreads), 0, st, rho, G, dom);
 }
@@ -1267,8 +1487,8 @@ void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
 template void launch_fill<double>(double*, int64_t, hipStream_t);
 template void launch_fill<float>(float*, int64_t, hipStream_t);
 template void launch_deposit<double>(const double*, int64_t, const uint8_t*, const int32_t*,
-                                     int64_t, int, int, int, DomainRec*, double*, const StepCtl*,
-                                     hipStream_t);
+                                     bool, int64_t, int, int, int, DomainRec*, double*,
+                                     const StepCtl*, hipStream_t);
 template void launch_efield<double>(const double*, int, int, int, const DomainRec*, double4*,
                                     hipStream_t);
 template void launch_efield<float>(const float*, int, int, int, const DomainRec*, float4*,

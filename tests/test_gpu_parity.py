@@ -193,6 +193,39 @@ def test_deposit_lane_widths(gpu, dense, precision):
         assert err <= tol
 
 
+@pytest.mark.parametrize("precision", [sc.FP64, sc.FP32_GRID])
+def test_deposit_windows_out_of_cell_order(gpu, precision):
+    """rho when the bunch storage is no longer in cell order (the deposit's
+    windows are sorted by cell in LDS): one apply reorders the bunch, the
+    particles are then moved by several cells (and some masked out), and the
+    next apply (no reorder) must give the oracle's rho of the moved bunch."""
+    shape = (32, 32, 64)
+    n = 200003
+    parts = gaussian_parts(n, C1_SIGMA, seed=41)
+    masks = np.ones(n, np.uint8)
+    fixed = ((0.0, 0.0, 0.0), (8e-3, 8e-3, 0.1))
+    dt = 0.1 / (BETA * sc.C_LIGHT)
+    op, b = gpu_run(make_opts(shape, sort_period=8, fixed=fixed, precision=precision), parts,
+                    masks, 5e10, dt)
+    moved, mm = b.host()  # now in cell order
+    rng = np.random.default_rng(12)
+    h = (8e-3 / 32, 8e-3 / 32, 0.1 / 64)
+    for col, hc in ((0, h[0]), (2, h[1]), (4, h[2])):
+        moved[col] += rng.normal(0.0, 3.0 * hc, n)
+    mm[rng.random(n) < 0.03] = 0
+    b.set_host(moved, mm)
+    op.apply_bunch(b, dt)
+    print(f"case windows precision={precision}", flush=True)
+    rc, out = oracle_run(oracle_kw(shape, fixed=fixed), moved, mm, 5e10, dt, want=("rho",))
+    assert rc == 0
+    err = rel_l2(op.debug(sc.DBG_RHO), out["rho"])
+    print("rho rel-L2", err)
+    # FP32: every cell is now split over many windows, so each node adds
+    # many once-rounded FP32 partial sums (2e-7 measured; the sorted bunch of
+    # test_fp32_grid_field_accuracy stays within 2e-7)
+    assert err <= (1e-13 if precision == sc.FP64 else 5e-7)
+
+
 def test_multi_step_parity(gpu):
     """Three kicks in a row (sort every 2nd apply) vs three oracle kicks."""
     n = 30000
