@@ -68,10 +68,12 @@ typedef struct sc3d_options {
     double kick_scale;           /* default 1                                */
     /* build extensions (no reference counterpart) */
     int32_t precision;           /* SC3D_PRECISION_*; default FP64           */
-    int32_t sort_period;         /* the particles are sorted by cell on every
-                                    apply; the bunch storage itself is
-                                    reordered into that order every N applies
-                                    (0 = never); default 8                   */
+    int32_t sort_period;         /* every N applies the particles are sorted
+                                    by cell and the bunch storage is reordered
+                                    into that order (the deposit sorts windows
+                                    of the storage on the device in between);
+                                    0 = never reorder: sort on every apply and
+                                    deposit through the permutation; default 8 */
 } sc3d_options;
 
 typedef struct sc3d_bunch {
