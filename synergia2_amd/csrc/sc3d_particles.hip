@@ -784,18 +784,29 @@ __device__ __forceinline__ void add_run(R* __restrict__ rho, int key, const Corn
 // the wavefront's combination (see above).  key[q] < 0 marks particles that
 // add nothing; those keys must differ from each other and from the keys of
 // every other lane of the wavefront.
-template <typename R, int K>
+// Run totals go to `emit(key, sums, q_last)`, q_last = the lane index of the
+// run's last particle in this lane (K-1 for a run that goes on into the next
+// lane or wavefront).
+template <typename R> struct AtomicRuns {
+    R* __restrict__ rho;
+    int gx, gy, gz;
+    __device__ __forceinline__ void operator()(int key, const CornerSums<R>& a, int) const
+    {
+        add_run<R>(rho, key, a, gx, gy, gz);
+    }
+};
+
+template <typename R, int K, typename Emit>
 __device__ __forceinline__ void deposit_runs(const int (&key)[K], const double (&ox)[K],
                                              const double (&oy)[K], const double (&oz)[K],
-                                             double w0, R* __restrict__ rho, int gx, int gy,
-                                             int gz)
+                                             double w0, const Emit& emit)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
     // runs inside the lane: the leading run (head), the trailing run (cur at
     // the end); runs in between are added at once
     CornerSums<R> cur, head;
-    int cur_key = 0, head_key = 0, cur_start = 0;
+    int cur_key = 0, head_key = 0, cur_start = 0, head_last = K - 1;
     bool single = true;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -809,9 +820,10 @@ __device__ __forceinline__ void deposit_runs(const int (&key)[K], const double (
             if (cur_start == 0) {
                 head = cur;
                 head_key = cur_key;
+                head_last = q - 1;
                 single = false;
             } else if (cur_key >= 0) {
-                add_run<R>(rho, cur_key, cur, gx, gy, gz);
+                emit(cur_key, cur, q - 1);
             }
             cur = p;
             cur_key = key[q];
@@ -837,10 +849,9 @@ __device__ __forceinline__ void deposit_runs(const int (&key)[K], const double (
     const int next_head = __shfl_down(head_key, 1, 64);
     if (!single && head_key >= 0) {
         if (lane > 0 && prev_tail == head_key) head.add(carry);
-        add_run<R>(rho, head_key, head, gx, gy, gz);
+        emit(head_key, head, head_last);
     }
-    if (cur_key >= 0 && (lane == 63 || next_head != cur_key))
-        add_run<R>(rho, cur_key, cur, gx, gy, gz);
+    if (cur_key >= 0 && (lane == 63 || next_head != cur_key)) emit(cur_key, cur, K - 1);
 }
 
 // Deposit through the sort permutation of this step (every apply sorts;
@@ -909,7 +920,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
                 key[q] = (iz * gy + iy) * gx + ix;
         }
     }
-    deposit_runs<R, K>(key, ox, oy, oz, dom->w0, rho, gx, gy, gz);
+    deposit_runs<R, K>(key, ox, oy, oz, dom->w0, AtomicRuns<R>{rho, gx, gy, gz});
 }
 
 // Deposit in windows of the bunch storage.  The storage is cell-sorted at
@@ -925,6 +936,10 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // offsets held in LDS 158 / 590; 1024 re-reading the coordinates 141 / 493;
 // 2048 re-reading 137 / 530; 4096 re-reading 166 / 726.
 constexpr int kWindow = 1024;
+#ifndef SC3D_RUN_CAP
+#define SC3D_RUN_CAP 512
+#endif
+constexpr int kRunCap = SC3D_RUN_CAP;  // runs per window held in LDS (more: direct atomics)
 
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_deposit_local(const double* __restrict__ parts,
@@ -935,8 +950,15 @@ __global__ __launch_bounds__(kThreads) void k_deposit_local(const double* __rest
                                                             R* __restrict__ rho)
 {
     constexpr int W = kWindow, PER = kWindow / kThreads, kWaves = kThreads / 64;
-    __shared__ int32_t skey[2][W];
-    __shared__ int32_t sslot[2][W];
+    // the sort buffers and, after the sort, the window's run list share LDS
+    constexpr int kSortBytes = 4 * W * int(sizeof(int32_t));
+    constexpr int kListBytes = kRunCap * int(sizeof(int32_t) + 8 * sizeof(R));
+    constexpr int kBytes = kSortBytes > kListBytes ? kSortBytes : kListBytes;
+    __shared__ __align__(16) unsigned char lds[kBytes];
+    int32_t(*skey)[W] = reinterpret_cast<int32_t(*)[W]>(lds);
+    int32_t(*sslot)[W] = reinterpret_cast<int32_t(*)[W]>(lds + 2 * W * sizeof(int32_t));
+    R(*rw)[8] = reinterpret_cast<R(*)[8]>(lds);
+    int32_t* rkey = reinterpret_cast<int32_t*>(lds + kRunCap * 8 * sizeof(R));
     __shared__ int32_t wave_off[kWaves][kMaxBuckets];
     __shared__ int32_t start_d[kMaxBuckets];
     __shared__ int32_t wsum[kWaves];
@@ -1064,28 +1086,78 @@ __global__ __launch_bounds__(kThreads) void k_deposit_local(const double* __rest
         cur ^= 1;
     }
     __syncthreads();
-    // in chunks of 4 * kThreads sorted positions: lane t takes [4t, 4t + 4)
+    // lane t takes sorted positions [4t, 4t + 4) (kWindow = 4 kThreads)
+    static_assert(PER == 4, "one chunk of 4 positions per lane");
+    int key[4], sl[4];
+    bool end[4];
 #pragma unroll
-    for (int h = 0; h < PER / 4; ++h) {
-        int key[4];
-        double ox[4], oy[4], oz[4];
+    for (int q = 0; q < 4; ++q) {
+        sl[q] = sslot[cur][4 * t + q] & (W - 1);
+        key[q] = skey[cur][4 * t + q];
+    }
+    const int after = 4 * t + 4 < W ? skey[cur][4 * t + 4] : sentinel;
+    // run ends in this lane (the sentinel run excluded): run r of the window
+    // is the one with r run ends before it
+    int ends = 0, before[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int pos = h * 4 * kThreads + 4 * t + q;
-            const int k = skey[cur][pos];
-            const int sl = sslot[cur][pos] & (W - 1);
-            key[q] = k != sentinel ? k : -1 - (lane * 4 + q);
-            // the window's coordinates again, in sorted order (cache hits;
-            // cheaper than holding the offsets in LDS, which costs occupancy)
-            ox[q] = oy[q] = oz[q] = 0.0;
-            if (k != sentinel) {
-                const int64_t j = base + sl;
-                cell_index(parts[j], lx, ihx, ox[q]);
-                cell_index(parts[j + 2 * stride], ly, ihy, oy[q]);
-                cell_index(parts[j + 4 * stride], lz, ihz, oz[q]);
-            }
+    for (int q = 0; q < 4; ++q) {
+        before[q] = ends;
+        end[q] = key[q] != sentinel && key[q] != (q < 3 ? key[q + 1] : after);
+        ends += end[q];
+    }
+    int32_t nruns;
+    const int32_t base_run = block_exclusive_scan(ends, wsum, nruns);
+    const bool listed = nruns <= kRunCap;
+    __syncthreads();  // the sort buffers are read: the run list may overwrite them
+    if (listed)
+        for (int e = t; e < nruns * 8; e += kThreads) rw[e >> 3][e & 7] = R(0);
+    __syncthreads();
+    double ox[4], oy[4], oz[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        // the window's coordinates again, in sorted order (cache hits;
+        // cheaper than holding the offsets in LDS, which costs occupancy)
+        ox[q] = oy[q] = oz[q] = 0.0;
+        if (key[q] != sentinel) {
+            const int64_t j = base + sl[q];
+            cell_index(parts[j], lx, ihx, ox[q]);
+            cell_index(parts[j + 2 * stride], ly, ihy, oy[q]);
+            cell_index(parts[j + 4 * stride], lz, ihz, oz[q]);
+        } else {
+            key[q] = -1 - (lane * 4 + q);
         }
-        deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, rho, gx, gy, gz);
+    }
+    if (!listed) {
+        deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, AtomicRuns<R>{rho, gx, gy, gz});
+        return;
+    }
+    // runs into the list (a run cut by a wavefront edge arrives in parts)
+    auto to_list = [&](int k, const CornerSums<R>& a, int q_last) {
+        const int r = base_run + before[q_last];
+        rkey[r] = k;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) atomicAdd(&rw[r][c], R(a.total(c)));
+    };
+    deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, to_list);
+    __syncthreads();
+    // one addition per node row and cell: corner (0, cy, cz) of cell k also
+    // takes corner (1, cy, cz) of cell k - 1 when that cell is in the list
+    // (same row: k - 1 of a cell with ix = 0 is no cell of the grid)
+    for (int r = t; r < nruns; r += kThreads) {
+        const int k = rkey[r];
+        const bool left = r > 0 && rkey[r - 1] == k - 1;
+        const bool right = r + 1 < nruns && rkey[r + 1] == k + 1;
+        const int ix = k % gx, rr = k / gx;
+        const int iy = rr % gy, iz = rr / gy;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if ((c & 1) && right) continue;  // added with cell k + 1
+            R v = rw[r][c];
+            if (!(c & 1) && left) v += rw[r - 1][c + 1];
+            const int cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1)
+                unsafeAtomicAdd(rho + (int64_t(cz) * gy + cy) * gx + cx, v);
+        }
     }
 }
 
