@@ -73,7 +73,7 @@ typedef struct sc3d_options {
                                     into that order (the deposit sorts windows
                                     of the storage on the device in between);
                                     0 = never reorder: sort on every apply and
-                                    deposit through the permutation; default 8 */
+                                    deposit through the permutation; default 12*/
 } sc3d_options;
 
 typedef struct sc3d_bunch {

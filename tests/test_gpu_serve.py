@@ -56,7 +56,7 @@ def close(a, b, tol):
 @pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-10), (sc.FP32_GRID, 1e-6)])
 def test_served_job_matches_operator_api(gpu, precision, tol):
     """One job in the server == the bunch tracked by the Propagator over the
-    operator API with the same options (sort_period 8)."""
+    operator API with the same options (same sort_period)."""
     job = SV.Job(30000, (32, 32, 64), 23, 1.5e-3, 1.0e-3, 0.012, seed=11)
     srv = SV.Server(SV.options(precision=precision, keep_particles=True))
     jid = run_all(srv, [job])[0]
