@@ -478,7 +478,14 @@ int sc3d_debug_get(sc3d_ctx* c, int what, void* dst, int64_t bytes)
     }
     case SC3D_DBG_GHAT:
         if (!need(g.Ghat_elems * 8)) return fail(SC3D_ERR_SIZE, "sc3d_debug_get: too small");
-        HIP_TRY(hipMemcpy(dst, c->ghat64, g.Ghat_elems * 8, hipMemcpyDeviceToHost));
+        if (c->fp32) {  // the precision form keeps the spectrum's FP32 copy only
+            std::vector<float> tmp(g.Ghat_elems);
+            HIP_TRY(hipMemcpy(tmp.data(), c->ghat32, g.Ghat_elems * 4, hipMemcpyDeviceToHost));
+            double* out = static_cast<double*>(dst);
+            for (int64_t i = 0; i < g.Ghat_elems; ++i) out[i] = tmp[i];
+        } else {
+            HIP_TRY(hipMemcpy(dst, c->ghat64, g.Ghat_elems * 8, hipMemcpyDeviceToHost));
+        }
         return SC3D_OK;
     default: return fail(SC3D_ERR_ARG, "sc3d_debug_get: unknown tap");
     }

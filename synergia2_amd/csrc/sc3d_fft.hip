@@ -837,27 +837,23 @@ __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restri
     load_twiddles<double2, L>(twl, tw);
     __syncthreads();
     lds_fft<double2, L, NL, false>(buf, twl);
+    // the last (z) pass of the precision form stores the spectrum rounded to
+    // FP32 only (the grid passes read that copy)
+    float* __restrict__ g32 =
+        AXIS == 2 && J.ghat != static_cast<void*>(J.ghat64) ? static_cast<float*>(J.ghat)
+                                                             : nullptr;
     for (int e = threadIdx.x; e < 2 * NL * (L / 2 + 1); e += kThreads) {
         int p = e / (2 * NL), q = e % (2 * NL);
         int c = 2 * l0 + q;
         if (c < ncols) {
             const double2 z = buf[LdsLine<L>::at((q >> 1), p)];
-            ghat[addr(c, p)] = (q & 1) ? z.y : z.x;
+            const double v = (q & 1) ? z.y : z.x;
+            if (g32)
+                g32[addr(c, p)] = float(v);
+            else
+                ghat[addr(c, p)] = v;
         }
     }
-}
-
-template <typename R>
-__global__ void k_cast_ghat(const GridJob* __restrict__ jobs, int64_t n)
-{
-    const GridJob& J = jobs[blockIdx.y];
-    const double* __restrict__ src = J.ghat64;
-    R* __restrict__ dst = static_cast<R*>(J.ghat);
-    const DomainRec* dom = J.dom;
-    if (dom->error || !dom->ghat_dirty) return;
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * blockDim.x)
-        dst[i] = R(src[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -925,9 +921,6 @@ void launch_green(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
     dispatch_len<GreenX>(g.gx, g, green_fn, jobs, nj, tw64, st);
     dispatch_len<GreenY>(g.Dy, g, jobs, nj, tw64, st);
     dispatch_len<GreenZ>(g.Dz, g, jobs, nj, tw64, st);
-    if constexpr (!std::is_same<R, double>::value) {
-        hipLaunchKernelGGL(k_cast_ghat<R>, dim3(256, nj), dim3(256), 0, st, jobs, g.Ghat_elems);
-    }
 }
 
 template <typename R> struct SolveX {
