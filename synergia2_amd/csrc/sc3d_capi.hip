@@ -185,38 +185,37 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
     a.ref_gamma = b->ref_gamma;
     a.ref_beta = b->ref_beta;
     // The bunch storage is cell-sorted (sort + reorder) every sort_period
-    // applies (decided here, or on the device under step control); the
-    // deposit then works on windows of the storage, sorted by cell in LDS.
-    // sort_period 0 never reorders: the sort then runs on every apply and the
-    // deposit walks its permutation.
+    // applies (served form: on the iterations the server picks, sc.sort_now);
+    // the deposit then works on windows of the storage, sorted by cell in
+    // LDS.  sort_period 0 never reorders: the sort then runs on every apply
+    // and the deposit walks its permutation.
     const bool local_dep = o.sort_period > 0;
     const bool do_reorder =
-        n > 0 && local_dep && (sc.ctl != nullptr || (c->applies % o.sort_period) == 0);
+        n > 0 && local_dep &&
+        (sc.ctl != nullptr ? sc.sort_now != 0 : (c->applies % o.sort_period) == 0);
     const bool do_sort = n > 0 && (do_reorder || !local_dep);
     ev[0] = mark(c, st);
     launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, sc, st);
     ev[1] = mark(c, st);
     const int32_t* dep_perm = nullptr;
     if (do_sort) {
-        // under step control the sort kernels skip the steps without a reorder
-        launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb,
-                    local_dep ? sc.ctl : nullptr, st);
+        launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb, st);
         c->sorted_n = n;
         if (do_reorder)
             launch_reorder(b->parts, b->stride, b->masks, n, c->sb.perm, c->scratch, c->smask,
-                           c->dom, sc.ctl, st);
+                           c->dom, st);
         if (!local_dep) dep_perm = c->sb.perm;
     }
     if (!c->fp32) {
         launch_fill<double>(static_cast<double*>(c->rho), g.G, st);
         ev[2] = mark(c, st);
         launch_deposit<double>(b->parts, b->stride, b->masks, dep_perm, local_dep, n, g.gx, g.gy,
-                               g.gz, c->dom, static_cast<double*>(c->rho), sc.ctl, st);
+                               g.gz, c->dom, static_cast<double*>(c->rho), st);
     } else {
         launch_fill<float>(static_cast<float*>(c->rho), g.G, st);
         ev[2] = mark(c, st);
         launch_deposit<float>(b->parts, b->stride, b->masks, dep_perm, local_dep, n, g.gx, g.gy,
-                              g.gz, c->dom, static_cast<float*>(c->rho), sc.ctl, st);
+                              g.gz, c->dom, static_cast<float*>(c->rho), st);
     }
     ev[3] = mark(c, st);
     return SC3D_OK;

@@ -54,16 +54,13 @@ __device__ __forceinline__ int cell_index(double pos, double left, double inv_h,
 
 // Device-side control of the applies of one context in the served form
 // (kernels take it as a nullable pointer; null = the host decides, as in
-// sc3d_apply).  k_step_begin opens each apply: whether it reorders the bunch
-// storage (every sort_period applies from the first), which scheduled map
-// programs run before / after the kick and which kick time step applies,
-// and whether the job has finished (then every kernel of the apply exits).
-// The launch sequence of an apply is then the same every step, so one
-// captured graph serves every step.
+// sc3d_apply).  k_step_begin opens each apply: which scheduled map programs
+// run before / after the kick and which kick time step applies, and whether
+// the job has finished (then every kernel of the apply exits).  The launch
+// sequence of an apply is then the same every step, so captured graphs
+// serve every step (the server picks the one with or without the reorder).
 struct StepCtl {
     int64_t applies;      // applies opened so far
-    int32_t sort_period;
-    int32_t reorder;      // this apply reorders the bunch storage
     int32_t turn_steps;   // schedule length (steps per turn)
     int32_t total_steps;  // steps of the job
     int32_t cur;          // schedule index of this apply

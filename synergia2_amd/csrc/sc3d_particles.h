@@ -26,6 +26,7 @@ struct Sched {
     const StepCtl* ctl;
     const MapProgram* progs;  // [first | between | last] x turn_steps
     const double* dt;         // kick time step per schedule entry
+    int32_t sort_now;         // served form: this iteration sorts and reorders the bunch
 };
 
 struct PopulateArgs {
@@ -57,14 +58,10 @@ void launch_step_begin(StepCtl* ctl, hipStream_t st);
 void launch_maps_sched(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                        const Sched& sc, hipStream_t st);
 int sort_passes(int64_t G);
-// ctl (served form): the sort kernels run only on the steps the device marks
-// for a reorder.
 void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
-                 int gy, int gz, const DomainRec* dom, SortBuffers& b, const StepCtl* ctl,
-                 hipStream_t st);
+                 int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st);
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
-                    double* scratch, uint8_t* smask, const DomainRec* dom, const StepCtl* ctl,
-                    hipStream_t st);
+                    double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st);
 template <typename R>
 void launch_fill(R* p, int64_t n, hipStream_t st);
 // Deposit: local (k_deposit_local): windows of the bunch storage, sorted by
@@ -75,11 +72,11 @@ void launch_fill(R* p, int64_t n, hipStream_t st);
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
                     const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
-                    DomainRec* dom, R* rho, const StepCtl* ctl, hipStream_t st);
+                    DomainRec* dom, R* rho, hipStream_t st);
 template <>
 void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* masks,
                            const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
-                           DomainRec* dom, float* rho, const StepCtl* ctl, hipStream_t st);
+                           DomainRec* dom, float* rho, hipStream_t st);
 template <typename R>
 void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
                    typename vec4<R>::type* E, hipStream_t st);

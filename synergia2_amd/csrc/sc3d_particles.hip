@@ -24,7 +24,6 @@ __global__ void k_step_begin(StepCtl* __restrict__ ctl)
         return;
     }
     c.done = 0;
-    c.reorder = c.sort_period > 0 && (j % c.sort_period) == 0;
     if (c.turn_steps > 0) {
         const int k = int(j % c.turn_steps);
         c.cur = k;
@@ -427,13 +426,12 @@ template <bool KEYS>
 __global__ __launch_bounds__(kThreads) void k_radix_count(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks, int gx,
     int gy, int gz, int32_t* __restrict__ keys, int64_t n, int shift, int width,
-    int32_t* __restrict__ counts, int nblk, const DomainRec* __restrict__ dom,
-    const StepCtl* __restrict__ ctl)
+    int32_t* __restrict__ counts, int nblk, const DomainRec* __restrict__ dom)
 {
     constexpr int kWaves = kThreads / 64;
     constexpr int kWaveItems = kSortTile / kWaves;
     __shared__ int32_t hist[kWaves][kMaxBuckets];
-    if (dom->error || (ctl && !ctl->reorder)) return;
+    if (dom->error) return;
     const unsigned blk = part_block();
     if (blk >= unsigned(nblk)) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -483,11 +481,10 @@ __global__ __launch_bounds__(kThreads) void k_radix_count(
 // place) and the digit total.
 __global__ __launch_bounds__(kThreads) void k_radix_scan(int32_t* __restrict__ counts, int nblk,
                                                          int32_t* __restrict__ totals,
-                                                         const DomainRec* dom,
-                                                         const StepCtl* __restrict__ ctl)
+                                                         const DomainRec* dom)
 {
     __shared__ int32_t wsum[kThreads / 64];
-    if (dom->error || (ctl && !ctl->reorder)) return;
+    if (dom->error) return;
     const int d = blockIdx.x;
     const int per = (nblk + kThreads - 1) / kThreads;
     const int b0 = threadIdx.x * per;
@@ -520,7 +517,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in, int64_t n,
     int shift, int width, const int32_t* __restrict__ counts, const int32_t* __restrict__ totals,
     int nblk, int32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
-    const DomainRec* dom, const StepCtl* __restrict__ ctl)
+    const DomainRec* dom)
 {
     constexpr int kWaves = kThreads / 64;
     constexpr int kWaveItems = kSortTile / kWaves;
@@ -530,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     __shared__ int32_t wsum[2][kWaves];
     __shared__ int32_t lkey[kSortTile];
     __shared__ int32_t lval[kSortTile];
-    if (dom->error || (ctl && !ctl->reorder)) return;
+    if (dom->error) return;
     const unsigned blk = part_block();
     if (blk >= unsigned(nblk)) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -613,9 +610,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 __global__ __launch_bounds__(kThreads) void k_gather_particles(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
     const int32_t* __restrict__ perm, int64_t n, double* __restrict__ scratch,
-    uint8_t* __restrict__ smask, const DomainRec* dom, const StepCtl* __restrict__ ctl)
+    uint8_t* __restrict__ smask, const DomainRec* dom)
 {
-    if (dom->error || (ctl && !ctl->reorder)) return;
+    if (dom->error) return;
     for (int64_t j = int64_t(part_block()) * kThreads + threadIdx.x; j < n;
          j += int64_t(gridDim.x) * kThreads) {
         int64_t src = perm[j];
@@ -630,10 +627,9 @@ __global__ __launch_bounds__(kThreads) void k_copy_back(double* __restrict__ par
                                                         uint8_t* __restrict__ masks, int64_t n,
                                                         const double* __restrict__ scratch,
                                                         const uint8_t* __restrict__ smask,
-                                                        const DomainRec* dom,
-                                                        const StepCtl* __restrict__ ctl)
+                                                        const DomainRec* dom)
 {
-    if (dom->error || (ctl && !ctl->reorder)) return;
+    if (dom->error) return;
     for (int64_t j = int64_t(part_block()) * kThreads + threadIdx.x; j < n;
          j += int64_t(gridDim.x) * kThreads) {
 #pragma unroll
@@ -863,12 +859,10 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
                                                       const int32_t* __restrict__ perm_in,
                                                       int64_t n, int gx, int gy, int gz,
                                                       const DomainRec* __restrict__ dom,
-                                                      R* __restrict__ rho,
-                                                      const StepCtl* __restrict__ ctl)
+                                                      R* __restrict__ rho)
 {
     if (dom->error) return;
-    // served form: right after a reorder the bunch itself is in cell order
-    const int32_t* __restrict__ perm = (ctl && ctl->reorder) ? nullptr : perm_in;
+    const int32_t* __restrict__ perm = perm_in;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
     const int lane = threadIdx.x & 63;
@@ -1419,8 +1413,7 @@ static int sort_bits(int64_t G)
 }
 
 void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx,
-                 int gy, int gz, const DomainRec* dom, SortBuffers& b, const StepCtl* ctl,
-                 hipStream_t st)
+                 int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st)
 {
     const int nblk = int((n + kSortTile - 1) / kSortTile);
     const int bits = sort_bits(int64_t(gx) * gy * gz);
@@ -1436,17 +1429,17 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
         if (p == 0)
             hipLaunchKernelGGL(k_radix_count<true>, dim3(round8(nblk)), dim3(kThreads), 0, st,
                                parts, stride, masks, gx, gy, gz, b.keys0, n, shift, width,
-                               b.counts, nblk, dom, ctl);
+                               b.counts, nblk, dom);
         else
             hipLaunchKernelGGL(k_radix_count<false>, dim3(round8(nblk)), dim3(kThreads), 0, st,
                                (const double*)nullptr, int64_t(0), (const uint8_t*)nullptr, gx,
                                gy, gz, const_cast<int32_t*>(kin), n, shift, width, b.counts,
-                               nblk, dom, ctl);
+                               nblk, dom);
         hipLaunchKernelGGL(k_radix_scan, dim3(nb), dim3(kThreads), 0, st, b.counts, nblk,
-                           b.totals, dom, ctl);
+                           b.totals, dom);
         hipLaunchKernelGGL(k_radix_scatter, dim3(round8(nblk)), dim3(kThreads), 0, st, kin, vin,
                            n, shift, width, b.counts, b.totals, nblk, kout[p & 1], vout[p & 1],
-                           dom, ctl);
+                           dom);
         kin = kout[p & 1];
         vin = vout[p & 1];
     }
@@ -1455,13 +1448,12 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
 }
 
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
-                    double* scratch, uint8_t* smask, const DomainRec* dom, const StepCtl* ctl,
-                    hipStream_t st)
+                    double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st)
 {
     hipLaunchKernelGGL(k_gather_particles, dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                       stride, masks, perm, n, scratch, smask, dom, ctl);
+                       stride, masks, perm, n, scratch, smask, dom);
     hipLaunchKernelGGL(k_copy_back, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride,
-                       masks, n, scratch, smask, dom, ctl);
+                       masks, n, scratch, smask, dom);
 }
 
 template <typename R>
@@ -1479,7 +1471,7 @@ void launch_fill(R* p, int64_t n, hipStream_t st)
 template <typename R>
 static void launch_deposit_k(const double* parts, int64_t stride, const uint8_t* masks,
                              const int32_t* perm, int64_t n, int gx, int gy, int gz,
-                             DomainRec* dom, R* rho, const StepCtl* ctl, hipStream_t st)
+                             DomainRec* dom, R* rho, hipStream_t st)
 {
     const int64_t G = int64_t(gx) * gy * gz;
     const int K = n >= 6 * G ? 4 : 2;
@@ -1487,10 +1479,10 @@ static void launch_deposit_k(const double* parts, int64_t stride, const uint8_t*
     if (nb < 1) return;
     if (K == 4)
         hipLaunchKernelGGL((k_deposit<R, 4>), dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
-                           stride, masks, perm, n, gx, gy, gz, dom, rho, ctl);
+                           stride, masks, perm, n, gx, gy, gz, dom, rho);
     else
         hipLaunchKernelGGL((k_deposit<R, 2>), dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
-                           stride, masks, perm, n, gx, gy, gz, dom, rho, ctl);
+                           stride, masks, perm, n, gx, gy, gz, dom, rho);
 }
 
 template <typename R>
@@ -1507,23 +1499,23 @@ static void launch_deposit_local(const double* parts, int64_t stride, const uint
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
                     const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
-                    DomainRec* dom, R* rho, const StepCtl* ctl, hipStream_t st)
+                    DomainRec* dom, R* rho, hipStream_t st)
 {
     if (local)
         launch_deposit_local<R>(parts, stride, masks, n, gx, gy, gz, dom, rho, st);
     else
-        launch_deposit_k<R>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, ctl, st);
+        launch_deposit_k<R>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, st);
 }
 
 template <>
 void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* masks,
                            const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,
-                           DomainRec* dom, float* rho, const StepCtl* ctl, hipStream_t st)
+                           DomainRec* dom, float* rho, hipStream_t st)
 {
     if (local)
         launch_deposit_local<float>(parts, stride, masks, n, gx, gy, gz, dom, rho, st);
     else
-        launch_deposit_k<float>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, ctl, st);
+        launch_deposit_k<float>(parts, stride, masks, perm, n, gx, gy, gz, dom, rho, st);
     const int64_t G = int64_t(gx) * gy * gz;
     hipLaunchKernelGGL(k_rho_peak, dim3(grid_for(G / 8, 512)), dim3(kThreads), 0, st, rho, G, dom);
 }
@@ -1560,7 +1552,7 @@ template void launch_fill<double>(double*, int64_t, hipStream_t);
 template void launch_fill<float>(float*, int64_t, hipStream_t);
 template void launch_deposit<double>(const double*, int64_t, const uint8_t*, const int32_t*,
                                      bool, int64_t, int, int, int, DomainRec*, double*,
-                                     const StepCtl*, hipStream_t);
+                                     hipStream_t);
 template void launch_efield<double>(const double*, int, int, int, const DomainRec*, double4*,
                                     hipStream_t);
 template void launch_efield<float>(const float*, int, int, int, const DomainRec*, float4*,

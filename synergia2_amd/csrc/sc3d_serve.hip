@@ -5,12 +5,15 @@
 //   - the per-job particle passes spread over a few streams (independent
 //     branches), the grid passes batched over all same-shape jobs (one
 //     launch per pass, blockIdx.y = job),
-//   - the whole iteration captured once as a hipGraph and replayed while the
-//     resident set is unchanged: reorder, schedule entry, kick time step and
-//     "finished" are decided on the device by each job's StepCtl, so the
-//     launch sequence is the same every step; a finished job stays in the
-//     graph as a no-op until the next rebuild, which admissions trigger at
-//     most every admit_interval iterations.
+//   - the whole iteration captured as two hipGraphs, replayed while the
+//     resident set is unchanged: one sorts and reorders every bunch by cell
+//     (the first iteration after a rebuild, which brings newly admitted,
+//     unsorted bunches, and every sort_period iterations after it), the other
+//     does not; schedule entry, kick time step and "finished" are decided on
+//     the device by each job's StepCtl, so the launch sequence is the same
+//     every step; a finished job stays in the graphs as a no-op until the
+//     next rebuild, which admissions trigger at most every admit_interval
+//     iterations.
 // Each job's step is the operator's apply (sc3d_ctx pieces, as sc3d_apply)
 // between its scheduled maps, exactly as Propagator::do_step runs Step::apply
 // (simulation/propagator.cc:157-283, step.cc:44-78) for one bunch.
@@ -54,7 +57,7 @@ struct Job {
     MapProgram* d_progs = nullptr;
     double* d_dt = nullptr;
     sc3d_bunch b{};
-    Sched sched() const { return Sched{ctl, d_progs, d_dt}; }
+    Sched sched(bool sort_now) const { return Sched{ctl, d_progs, d_dt, sort_now ? 1 : 0}; }
 };
 
 struct Group {
@@ -132,8 +135,9 @@ struct sc3d_server {
     int64_t next_id = 1;
     int64_t used = 0;
     bool dirty = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // the iteration's graph, without and with the sort + reorder of every job
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     int64_t iters_since_build = 0;
     int64_t iterations = 0, graph_builds = 0, jobs_done = 0, jobs_submitted = 0;
     double particle_steps = 0.0, busy_s = 0.0;
@@ -178,10 +182,12 @@ void release_groups(sc3d_server* s)
 
 void release_graph(sc3d_server* s)
 {
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    s->exec = nullptr;
-    s->graph = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (s->exec[k]) (void)hipGraphExecDestroy(s->exec[k]);
+        if (s->graph[k]) (void)hipGraphDestroy(s->graph[k]);
+        s->exec[k] = nullptr;
+        s->graph[k] = nullptr;
+    }
 }
 
 int admit(sc3d_server* s, Job* j)
@@ -196,7 +202,6 @@ int admit(sc3d_server* s, Job* j)
     HIP_TRY(hipMalloc(&j->d_progs, j->progs.size() * sizeof(MapProgram)));
     HIP_TRY(hipMalloc(&j->d_dt, j->dt.size() * sizeof(double)));
     StepCtl c{};
-    c.sort_period = s->opt.sort_period;
     c.turn_steps = sp.turn_steps;
     c.total_steps = sp.steps;
     c.first_prog = -1;
@@ -251,8 +256,9 @@ int finish(sc3d_server* s, Job* j)
     return SC3D_OK;
 }
 
-// One step of every resident job, enqueued on s->st (and the branch streams).
-void enqueue_iteration(sc3d_server* s)
+// One step of every resident job, enqueued on s->st (and the branch streams);
+// sort_now: the jobs' bunches are also sorted by cell and reordered.
+void enqueue_iteration(sc3d_server* s, bool sort_now)
 {
     const int B = int(std::min<size_t>(s->aux.size(), std::max<size_t>(1, s->resident.size())));
     hipEvent_t ev[SC3D_NPHASES + 1];
@@ -270,7 +276,7 @@ void enqueue_iteration(sc3d_server* s)
     for (size_t i = 0; i < s->resident.size(); ++i) {
         Job* j = s->resident[i];
         hipStream_t q = s->aux[i % B];
-        const Sched sc = j->sched();
+        const Sched sc = j->sched(sort_now);
         launch_step_begin(j->ctl, q);
         launch_maps_sched(j->parts, j->b.stride, j->masks, j->b.n_active, sc, q);
         enqueue_front(j->ctx, &j->b, j->dt[0], sc, q, ev);
@@ -286,7 +292,7 @@ void enqueue_iteration(sc3d_server* s)
     fork(1);
     for (size_t i = 0; i < s->resident.size(); ++i) {
         Job* j = s->resident[i];
-        enqueue_kick(j->ctx, &j->b, j->sched(), s->aux[i % B]);
+        enqueue_kick(j->ctx, &j->b, j->sched(sort_now), s->aux[i % B]);
     }
     join(1);
 }
@@ -328,10 +334,12 @@ int rebuild(sc3d_server* s)
     s->dirty = false;
     s->iters_since_build = 0;
     if (!s->opt.use_graphs || s->resident.empty()) return SC3D_OK;
-    HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-    enqueue_iteration(s);
-    HIP_TRY(hipStreamEndCapture(s->st, &s->graph));
-    HIP_TRY(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(s, k == 1);
+        HIP_TRY(hipStreamEndCapture(s->st, &s->graph[k]));
+        HIP_TRY(hipGraphInstantiate(&s->exec[k], s->graph[k], nullptr, nullptr, 0));
+    }
     s->graph_builds++;
     return SC3D_OK;
 }
@@ -472,11 +480,16 @@ int sc3d_server_iterate(sc3d_server* s, int32_t* n_resident, int32_t* n_waiting)
     if (int(s->resident.size()) - act > std::max(2, act / 4)) s->dirty = true;
     if (s->dirty)
         if (int rc = rebuild(s)) return rc;
+    // the bunches are sorted and reordered on the first iteration after a
+    // rebuild (newly admitted bunches come unsorted) and every sort_period
+    // iterations after it
+    const bool sort_now =
+        s->opt.sort_period > 0 && s->iters_since_build % s->opt.sort_period == 0;
     const auto t_start = std::chrono::steady_clock::now();
-    if (s->exec) {
-        HIP_TRY(hipGraphLaunch(s->exec, s->st));
+    if (s->exec[sort_now ? 1 : 0]) {
+        HIP_TRY(hipGraphLaunch(s->exec[sort_now ? 1 : 0], s->st));
     } else {
-        enqueue_iteration(s);
+        enqueue_iteration(s, sort_now);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->st));
