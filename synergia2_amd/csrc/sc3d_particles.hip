@@ -653,16 +653,16 @@ __global__ void k_fill(R* __restrict__ p, int64_t n, R v)
 // Target: the G octant of rho2 (zyx, gx fastest).
 //
 // The eight corner weights are summed over runs of equal cell keys before
-// any global atomic.  Every lane takes K consecutive particles of the
-// cell-sorted order (through the permutation of this step's sort, or the
-// bunch itself right after a reorder; all 4K loads of a lane are issued
-// together) and sums its own runs in order; a run that starts and ends
-// inside the lane is added to rho at once.  The lanes' trailing runs are
+// any global atomic.  Every lane takes K consecutive particles of a
+// cell-sorted order (a window of the bunch storage sorted by cell in LDS,
+// k_deposit_local; or, when the storage is never reordered, this step's
+// sort permutation, k_deposit) and sums its own runs in order; a run that
+// starts and ends inside the lane is emitted at once.  The lanes' trailing runs are
 // then summed across lanes by one segmented inclusive scan (a segment
 // continues through lanes that hold a single run of the same key), and the
 // total reaching a lane whose leading run has that key is carried into it.
-// Every run is added once, by the lane where it ends.  Any particle order is
-// correct; a cell-sorted bunch gives about one addition per cell and
+// Every run is emitted once, by the lane where it ends.  Any particle order
+// is correct; a cell-sorted order gives about one run per cell and
 // wavefront.  Lanes past n and particles that are masked out or off the
 // grid break runs and add nothing.
 //
@@ -923,12 +923,17 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // cells: the block loads its window coalesced, sorts (key, slot) by cell key
 // in LDS (stable LSD radix over the bits of key - min key of the window,
 // <= 8-bit digits, wave-private ranking as in the global scatter; skipped
-// when the window is already in cell order), and deposits it in that order
-// with the lane runs (4 particles per lane).  A cell whose particles fall in
-// several windows is added once per window.  Measured at 4e6 / 1.6e7
-// particles (deposit phase, us, sort_period 8): windows of 1024 with the
-// offsets held in LDS 158 / 590; 1024 re-reading the coordinates 141 / 493;
-// 2048 re-reading 137 / 530; 4096 re-reading 166 / 726.
+// when the window is already in cell order), and sums it in that order with
+// the lane runs (4 particles per lane).  The runs go into an LDS list of the
+// window (key order, over the sort buffers); there each cell adds its four
+// +x corners together with the -x corners of the next cell when that cell
+// is its row neighbour, so shared nodes take one atomic instead of two (a
+// window with more than kRunCap runs adds its runs directly).  A cell whose
+// particles fall in several windows is added once per window.  Measured at
+// 4e6 / 1.6e7 particles (deposit phase, us, sort_period 8, before the run
+// list): windows of 1024 with the offsets held in LDS 158 / 590; 1024
+// re-reading the coordinates 141 / 493; 2048 re-reading 137 / 530; 4096
+// re-reading 166 / 726.  The run list: 104 / 455.
 constexpr int kWindow = 1024;
 #ifndef SC3D_RUN_CAP
 #define SC3D_RUN_CAP 512
