@@ -8,9 +8,9 @@ an O-F-O-D cell (0.5 m drifts, 0.2 m quads k1 = +-0.6 m^-2, rms emittance
 grid 64x64x128 (doubled 128x128x256), adaptive domain, linear Green's
 function, FP64 throughout.
 
-A step = one space-charge step (the hot path: bunch extent, cell sort,
-deposit, Green's spectrum, FFT convolution, E field, kick) of a bunch that is
-tracked through the channel: the half-step drift/quadrupole maps run between
+A step = one space-charge step (the hot path: bunch extent, cell sort and
+reorder of the bunch every sort_period steps, deposit, Green's spectrum, FFT
+convolution, E field, kick) of a bunch that is tracked through the channel: the half-step drift/quadrupole maps run between
 the kicks as their own kernels.  value = macroparticles x steps / (sum of the
 HIP-event times of the timed steps, first kernel of the step to last, on the
 stream they run on) = particle-kicks/s.  The wall time of the whole timed
