@@ -88,7 +88,8 @@ def algorithmic_bytes(phase, n, g, w):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=48,
+                    help="timed steps (a multiple of the default sort period, 12)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="fodo_fp64_4M_64x64x128", choices=sorted(CONFIGS))
     ap.add_argument("--sort-period", type=int, default=None)
