@@ -65,13 +65,14 @@ template <int L, typename T2> struct ColLines {
 // block's LDS (K lines per transform line plus the twiddle table) exceeds
 // the cap: the passes are latency-bound at low occupancy, and gfx950 has
 // 160 KiB of LDS per CU.
-// (Measured at 64x64x128 and 128x128x256: 32 KiB for FP64 lines, 48 KiB
-// for FP32 lines.)
+// (Measured at 64x64x128 and 128x128x256: 32 KiB for FP64 lines (24 KiB
+// the same, 48 KiB slower); 32 KiB for FP32 lines (field passes 273 us vs
+// 280 us at 48 KiB, 303 us at 64 KiB).)
 #ifndef SC3D_LDS_CAP_FP64
 #define SC3D_LDS_CAP_FP64 32768
 #endif
 #ifndef SC3D_LDS_CAP_FP32
-#define SC3D_LDS_CAP_FP32 49152
+#define SC3D_LDS_CAP_FP32 32768
 #endif
 template <int L, int K, typename T2, int PREF> struct CappedLines {
     static constexpr size_t cap = sizeof(T2) >= 16 ? SC3D_LDS_CAP_FP64 : SC3D_LDS_CAP_FP32;
