@@ -1538,12 +1538,15 @@ void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                  int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
                  double m, const MapProgram& prog, const Sched& sc, hipStream_t st)
 {
+    // up to 32768 blocks (measured at 1.6e7 particles: 240 us, against 251
+    // with 8192 blocks, 272 with 4096 and 247 with one particle per thread)
+    const dim3 grid(grid_for(n, 32768));
     if (prog.n > 0 || sc.progs)
-        hipLaunchKernelGGL((k_kick<R, true>), dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog, sc);
+        hipLaunchKernelGGL((k_kick<R, true>), grid, dim3(kThreads), 0, st, parts, stride, masks,
+                           n, gx, gy, gz, dom, E, pref, m, prog, sc);
     else
-        hipLaunchKernelGGL((k_kick<R, false>), dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                           stride, masks, n, gx, gy, gz, dom, E, pref, m, prog, sc);
+        hipLaunchKernelGGL((k_kick<R, false>), grid, dim3(kThreads), 0, st, parts, stride, masks,
+                           n, gx, gy, gz, dom, E, pref, m, prog, sc);
 }
 
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
