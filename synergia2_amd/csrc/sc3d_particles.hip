@@ -1455,10 +1455,14 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
                     double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gather_particles, dim3(grid_for(n)), dim3(kThreads), 0, st, parts,
-                       stride, masks, perm, n, scratch, smask, dom);
-    hipLaunchKernelGGL(k_copy_back, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride,
-                       masks, n, scratch, smask, dom);
+    // One particle per thread up to 1.7e7 particles: the gather is bound by the latency of its
+    // random reads, so more waves in flight help (measured at 1.6e7 particles, one sort every
+    // 12 steps: sort plus reorder 132 us per step on average with 65536 blocks, 138 with 8192).
+    const dim3 grid(grid_for(n, 65536));
+    hipLaunchKernelGGL(k_gather_particles, grid, dim3(kThreads), 0, st, parts, stride, masks,
+                       perm, n, scratch, smask, dom);
+    hipLaunchKernelGGL(k_copy_back, grid, dim3(kThreads), 0, st, parts, stride, masks, n,
+                       scratch, smask, dom);
 }
 
 template <typename R>
