@@ -969,13 +969,18 @@ constexpr int kWindow = 1024;
 #endif
 constexpr int kRunCap = SC3D_RUN_CAP;  // runs per window held in LDS (more: direct atomics)
 
+// The kernel is latency-bound, so blocks per CU matter.  FP64: 40 KiB of LDS
+// and 118 VGPRs, both limits at 4 blocks per CU.  FP32: 24 KiB of LDS, so the
+// registers are held to 96 (5 waves per SIMD, a few values spilled): at 1.6e7
+// particles 405 us against 448 us at 109 VGPRs (4 waves).  Measured and not
+// kept: 6 waves (80 VGPRs, more spills) 477 us for FP32 and 125 us against 109
+// for FP64 at 4e6; 5 waves for FP64 (run list cut to 384 to fit the LDS)
+// 119 us.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_deposit_local(const double* __restrict__ parts,
-                                                            int64_t stride,
-                                                            const uint8_t* __restrict__ masks,
-                                                            int64_t n, int gx, int gy, int gz,
-                                                            const DomainRec* __restrict__ dom,
-                                                            R* __restrict__ rho)
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(float) ? 5 : 1))) void k_deposit_local(
+    const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
+    int64_t n, int gx, int gy, int gz, const DomainRec* __restrict__ dom, R* __restrict__ rho)
 {
     constexpr int W = kWindow, PER = kWindow / kThreads, kWaves = kThreads / 64;
     // the sort buffers and, after the sort, the window's run list share LDS
