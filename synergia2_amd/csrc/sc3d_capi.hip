@@ -122,7 +122,7 @@ int64_t ctx_footprint(const sc3d_options& o, int64_t n)
     b += g.S_elems * 2 * rs * (fp32 ? 2 : 1);    // S (+ S2)
     b += fp32 ? 0 : g.G * rs;                    // phi
     b += g.G * 4 * rs;                           // E
-    b += g.Ghat_elems * (8 + (fp32 ? 4 : 0));    // Green's spectrum (+ FP32 copy)
+    b += g.Ghat_elems * (fp32 ? 4 : 8);          // Green's spectrum
     b += kTwiddleElems * (16 + 8);               // twiddles
     const int64_t cap = n + n / 8 + 4096;        // ensure_particle_buffers
     const int64_t tiles = (cap + kSortTileItems - 1) / kSortTileItems;
@@ -238,7 +238,7 @@ void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream
         }
         ev[6] = mark(c, st);
     } else {
-        launch_green<float>(g, c->opt.green_fn, table, nj, c->tw64, st);
+        launch_green<float>(g, c->opt.green_fn, table, nj, c->tw32, st);
         ev[4] = mark(c, st);
         // the field comes straight out of the spectrum (no potential, no
         // separate difference pass): sc3d_force stays empty in this form
@@ -333,8 +333,10 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     }
     rc |= dmalloc(c, &p, g.G * 4 * rs);
     c->E = p;
-    rc |= dmalloc(c, &c->ghat64, g.Ghat_elems);
-    if (c->fp32) rc |= dmalloc(c, &c->ghat32, g.Ghat_elems);
+    if (c->fp32)
+        rc |= dmalloc(c, &c->ghat32, g.Ghat_elems);
+    else
+        rc |= dmalloc(c, &c->ghat64, g.Ghat_elems);
     rc |= dmalloc(c, &c->tw64, kTwiddleElems);
     rc |= dmalloc(c, &c->tw32, kTwiddleElems);
     rc |= dmalloc(c, &c->d_gj, 1);

@@ -89,9 +89,9 @@ using FieldLines = CappedLines<L, K, T2, ColLines<L, T2>::value>;
 // row passes holding K lines per row (rows are contiguous: no run width)
 template <int M, int K, typename T2>
 using RowLines = CappedLines<M, K, T2, Lines<M>::value>;
-// the Green's function column passes (FP64, two real columns per line)
-template <int L>
-using GreenLines = CappedLines<L, 1, double2, Lines<L>::value>;
+// the Green's function column passes (two real columns per line)
+template <int L, typename T2>
+using GreenLines = CappedLines<L, 1, T2, Lines<L>::value>;
 
 template <int R, bool INV, typename T2>
 __device__ __forceinline__ void dft_small(T2* v)
@@ -727,21 +727,33 @@ __device__ double green_linear(double x, double y, double z, double hz, double G
     return G;
 }
 
+// The spectrum is transformed in T: FP64 in the FP64 form, FP32 in the
+// precision form (whose convolution reads it in FP32).  The function values
+// are always evaluated in FP64 (2 s0 - sn - sp cancels) and rounded once.
+template <typename T> __device__ __forceinline__ T* green_buffer(const GridJob& J)
+{
+    if constexpr (sizeof(T) == sizeof(double))
+        return J.ghat64;
+    else
+        return static_cast<float*>(J.ghat);
+}
+
 // G1: rows (Z<=gz, Y<=gy) of the doubled grid: generate, R2C along x, keep
 // the real part of kx = 0..M (the row is even, its spectrum real).
-template <int M>
+template <typename T, int M>
 __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict__ jobs,
-                                                      const double2* __restrict__ twM,
-                                                      const double2* __restrict__ tw2M,
+                                                      const typename cplx<T>::type* __restrict__ twM,
+                                                      const typename cplx<T>::type* __restrict__ tw2M,
                                                       int gy, int gz, int green_fn)
 {
+    using T2 = typename cplx<T>::type;
     constexpr int NL = Lines<M>::value;
     constexpr int Dx = 2 * M;
-    __shared__ double2 buf[NL * LdsLine<M>::stride];
-    __shared__ double2 twl[M];
-    __shared__ double row[NL * (M + 1)];
+    __shared__ T2 buf[NL * LdsLine<M>::stride];
+    __shared__ T2 twl[M];
+    __shared__ T row[NL * (M + 1)];
     const GridJob& J = jobs[blockIdx.y];
-    double* __restrict__ ghat = J.ghat64;
+    T* __restrict__ ghat = green_buffer<T>(J);
     const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
     const int nrows = (gz + 1) * (gy + 1);
@@ -768,34 +780,34 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
                 v = (X == 0 && Y == 0 && Z == 0) ? G000 : 1.0 / sqrt(x * x + y * y + z * z);
             }
         }
-        row[e] = v;
+        row[e] = T(v);
     }
-    load_twiddles<double2, M>(twl, twM);
+    load_twiddles<T2, M>(twl, twM);
     __syncthreads();
     for (int e = threadIdx.x; e < NL * M; e += kThreads) {
         int line = e / M, n = e % M;
         int X0 = 2 * n, X1 = 2 * n + 1;
         int m0 = X0 <= M ? X0 : Dx - X0;
         int m1 = X1 <= M ? X1 : Dx - X1;
-        double2 v;
+        T2 v;
         v.x = row[line * (M + 1) + m0];
         v.y = row[line * (M + 1) + m1];
         buf[LdsLine<M>::at(line, n)] = v;
     }
     __syncthreads();
-    lds_fft<double2, M, NL, false>(buf, twl);
+    lds_fft<T2, M, NL, false>(buf, twl);
     const int Hx = M + 1;
     for (int e = threadIdx.x; e < NL * Hx; e += kThreads) {
         int line = e / Hx, k = e % Hx;
         int r = r0 + line;
         if (r >= nrows) continue;
-        double2 zk = buf[LdsLine<M>::at(line, (k == M ? 0 : k))];
-        double2 zm = buf[LdsLine<M>::at(line, (k == 0 ? 0 : M - k))];
-        double2 A, B;
+        T2 zk = buf[LdsLine<M>::at(line, (k == M ? 0 : k))];
+        T2 zm = buf[LdsLine<M>::at(line, (k == 0 ? 0 : M - k))];
+        T2 A, B;
         A.x = zk.x + zm.x; A.y = zk.y - zm.y;
         B.x = zk.x - zm.x; B.y = zk.y + zm.y;
-        double2 Cc = cmul(tw2M[k], B);
-        ghat[int64_t(r) * Hx + k] = 0.5 * (A.x + Cc.y);
+        T2 Cc = cmul(tw2M[k], B);
+        ghat[int64_t(r) * Hx + k] = T(0.5) * (A.x + Cc.y);
     }
 }
 
@@ -808,16 +820,17 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
 // imaginary parts.
 // AXIS 1 (y): address (plane*(gy+1) + p)*Hx + kx, plane = Z <= gz.
 // AXIS 2 (z): address (p*(gy+1) + plane)*Hx + kx, plane = ky <= gy.
-template <int L, int AXIS>
+template <typename T, int L, int AXIS>
 __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restrict__ jobs,
-                                                         const double2* __restrict__ tw,
+                                                         const typename cplx<T>::type* __restrict__ tw,
                                                          int ncols, int gy, int Hx)
 {
-    constexpr int NL = GreenLines<L>::value;
-    __shared__ double2 buf[NL * LdsLine<L>::stride];
-    __shared__ double2 twl[L];
+    using T2 = typename cplx<T>::type;
+    constexpr int NL = GreenLines<L, T2>::value;
+    __shared__ T2 buf[NL * LdsLine<L>::stride];
+    __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
-    double* __restrict__ ghat = J.ghat64;
+    T* __restrict__ ghat = green_buffer<T>(J);
     const DomainRec* dom = J.dom;
     if (dom->error || !dom->ghat_dirty) return;
     const int l0 = xcd_group() * NL;
@@ -831,28 +844,19 @@ __global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restri
     for (int e = threadIdx.x; e < 2 * NL * L; e += kThreads) {
         int p = e / (2 * NL), q = e % (2 * NL);
         int c = 2 * l0 + q;
-        double v = c < ncols ? ghat[addr(c, p <= L / 2 ? p : L - p)] : 0.0;
-        double2* slot = &buf[LdsLine<L>::at((q >> 1), p)];
+        T v = c < ncols ? ghat[addr(c, p <= L / 2 ? p : L - p)] : T(0);
+        T2* slot = &buf[LdsLine<L>::at((q >> 1), p)];
         if (q & 1) slot->y = v; else slot->x = v;
     }
-    load_twiddles<double2, L>(twl, tw);
+    load_twiddles<T2, L>(twl, tw);
     __syncthreads();
-    lds_fft<double2, L, NL, false>(buf, twl);
-    // the last (z) pass of the precision form stores the spectrum rounded to
-    // FP32 only (the grid passes read that copy)
-    float* __restrict__ g32 =
-        AXIS == 2 && J.ghat != static_cast<void*>(J.ghat64) ? static_cast<float*>(J.ghat)
-                                                             : nullptr;
+    lds_fft<T2, L, NL, false>(buf, twl);
     for (int e = threadIdx.x; e < 2 * NL * (L / 2 + 1); e += kThreads) {
         int p = e / (2 * NL), q = e % (2 * NL);
         int c = 2 * l0 + q;
         if (c < ncols) {
-            const double2 z = buf[LdsLine<L>::at((q >> 1), p)];
-            const double v = (q & 1) ? z.y : z.x;
-            if (g32)
-                g32[addr(c, p)] = float(v);
-            else
-                ghat[addr(c, p)] = v;
+            const T2 z = buf[LdsLine<L>::at((q >> 1), p)];
+            ghat[addr(c, p)] = (q & 1) ? z.y : z.x;
         }
     }
 }
@@ -882,46 +886,49 @@ inline int col_blocks(int64_t items, int per)
     return (nblocks(items, per) + kXcds - 1) / kXcds * kXcds;
 }
 
-template <int M> struct GreenX {
-    static void run(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
-                    const double2* tw, hipStream_t st)
-    {
-        int rows = (g.gz + 1) * (g.gy + 1);
-        hipLaunchKernelGGL(k_green_x<M>, dim3(nblocks(rows, Lines<M>::value), nj),
-                           dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
-                           tw + twiddle_offset(2 * M), g.gy, g.gz, green_fn);
-    }
-};
-template <int L> struct GreenY {
-    static void run(const Geometry& g, const GridJob* jobs, int nj, const double2* tw,
-                    hipStream_t st)
-    {
-        int cols = (g.gz + 1) * g.Hx;
-        hipLaunchKernelGGL((k_green_cols<L, 1>),
-                           dim3(col_blocks(cols, 2 * GreenLines<L>::value), nj), dim3(kThreads), 0,
-                           st,
-                           jobs, tw + twiddle_offset(L), cols, g.gy, g.Hx);
-    }
-};
-template <int L> struct GreenZ {
-    static void run(const Geometry& g, const GridJob* jobs, int nj, const double2* tw,
-                    hipStream_t st)
-    {
-        int cols = (g.gy + 1) * g.Hx;
-        hipLaunchKernelGGL((k_green_cols<L, 2>),
-                           dim3(col_blocks(cols, 2 * GreenLines<L>::value), nj), dim3(kThreads), 0,
-                           st,
-                           jobs, tw + twiddle_offset(L), cols, g.gy, g.Hx);
-    }
+template <typename T> struct Green {
+    using T2 = typename cplx<T>::type;
+    template <int M> struct X {
+        static void run(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
+                        const T2* tw, hipStream_t st)
+        {
+            int rows = (g.gz + 1) * (g.gy + 1);
+            hipLaunchKernelGGL((k_green_x<T, M>), dim3(nblocks(rows, Lines<M>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
+                               tw + twiddle_offset(2 * M), g.gy, g.gz, green_fn);
+        }
+    };
+    template <int L> struct Y {
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
+        {
+            int cols = (g.gz + 1) * g.Hx;
+            hipLaunchKernelGGL((k_green_cols<T, L, 1>),
+                               dim3(col_blocks(cols, 2 * GreenLines<L, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), cols, g.gy,
+                               g.Hx);
+        }
+    };
+    template <int L> struct Z {
+        static void run(const Geometry& g, const GridJob* jobs, int nj, const T2* tw,
+                        hipStream_t st)
+        {
+            int cols = (g.gy + 1) * g.Hx;
+            hipLaunchKernelGGL((k_green_cols<T, L, 2>),
+                               dim3(col_blocks(cols, 2 * GreenLines<L, T2>::value), nj),
+                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), cols, g.gy,
+                               g.Hx);
+        }
+    };
 };
 
 template <typename R>
 void launch_green(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
-                  const double2* tw64, hipStream_t st)
+                  const typename cplx<R>::type* tw, hipStream_t st)
 {
-    dispatch_len<GreenX>(g.gx, g, green_fn, jobs, nj, tw64, st);
-    dispatch_len<GreenY>(g.Dy, g, jobs, nj, tw64, st);
-    dispatch_len<GreenZ>(g.Dz, g, jobs, nj, tw64, st);
+    dispatch_len<Green<R>::template X>(g.gx, g, green_fn, jobs, nj, tw, st);
+    dispatch_len<Green<R>::template Y>(g.Dy, g, jobs, nj, tw, st);
+    dispatch_len<Green<R>::template Z>(g.Dz, g, jobs, nj, tw, st);
 }
 
 template <typename R> struct SolveX {
@@ -1038,7 +1045,7 @@ void launch_solve_field(const Geometry& g, const GridJob* jobs, int nj,
 
 template void launch_green<double>(const Geometry&, int, const GridJob*, int, const double2*,
                                    hipStream_t);
-template void launch_green<float>(const Geometry&, int, const GridJob*, int, const double2*,
+template void launch_green<float>(const Geometry&, int, const GridJob*, int, const float2*,
                                   hipStream_t);
 template void launch_solve<double>(const Geometry&, const GridJob*, int, const double2*,
                                    hipStream_t);

@@ -40,16 +40,16 @@ struct GridJob {
     void* S2;         // precision form: second spectrum workspace
     void* phi;        // FP64 form: R[G] potential octant
     void* E;          // vec4<R>[G] field (Ex, Ey, Ez, 0)
-    double* ghat64;   // Green's spectrum octant, FP64
-    void* ghat;       // the convolution's copy (FP32 form), == ghat64 in FP64
+    double* ghat64;   // FP64 form: Green's spectrum octant (null in the FP32 form)
+    void* ghat;       // the spectrum the convolution reads: FP32 octant, or ghat64
     DomainRec* dom;
 };
 
-// Green's spectrum (when the job's cell size changed) and, for R = float,
-// its FP32 copy.  All launches on `st`, asynchronous.
+// Green's spectrum (when the job's cell size changed), transformed in R
+// (values evaluated in FP64).  All launches on `st`, asynchronous.
 template <typename R>
 void launch_green(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
-                  const double2* tw64, hipStream_t st);
+                  const typename cplx<R>::type* tw, hipStream_t st);
 
 // FP64 form: rho -> phi (five passes, convolution fused into the z pass).
 template <typename R>
