@@ -54,6 +54,17 @@ template <int L> struct Lines {
     static constexpr int value = (1024 / L) > 8 ? (1024 / L) : 8;
 };
 
+// Half as many lines (>= 4, >= 512 elements) where more, smaller blocks pay:
+// the solve's x passes (rows are contiguous, so the line count does not set
+// the run width) and the Green's column passes.  These launches hold a few
+// blocks per CU at 64x64x128 and are latency-bound.  Measured (us per step,
+// 64x64x128 FP64 / 128x128x256 FP32): solve 59.4 -> 58.5 / 273 -> 265;
+// Green's function 35.5 -> 34.9 / 112 -> 111.  k_green_x keeps Lines (its
+// FP32 pass at 128x128x256: Green's phase 112 -> 119 us with half).
+template <int L> struct HalfLines {
+    static constexpr int value = (512 / L) > 4 ? (512 / L) : 4;
+};
+
 // Column passes: enough lines per block that each strided load or store
 // moves a 128-byte run (16 FP32 or 8 FP64 complex values).
 template <int L, typename T2> struct ColLines {
@@ -88,10 +99,10 @@ template <int L, int K, typename T2>
 using FieldLines = CappedLines<L, K, T2, ColLines<L, T2>::value>;
 // row passes holding K lines per row (rows are contiguous: no run width)
 template <int M, int K, typename T2>
-using RowLines = CappedLines<M, K, T2, Lines<M>::value>;
+using RowLines = CappedLines<M, K, T2, HalfLines<M>::value>;
 // the Green's function column passes (two real columns per line)
 template <int L, typename T2>
-using GreenLines = CappedLines<L, 1, T2, Lines<L>::value>;
+using GreenLines = CappedLines<L, 1, T2, HalfLines<L>::value>;
 
 template <int R, bool INV, typename T2>
 __device__ __forceinline__ void dft_small(T2* v)
@@ -251,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_x_r2c(const GridJob* __restrict__ 
                                                     int nrows, int gy, int Dy)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = Lines<M>::value;
+    constexpr int NL = HalfLines<M>::value;
     __shared__ T2 buf[NL * LdsLine<M>::stride];
     __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
@@ -317,7 +328,7 @@ __global__ __launch_bounds__(kThreads) void k_x_c2r(const GridJob* __restrict__ 
                                                     int nrows, int gy, int Dy)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = Lines<M>::value;
+    constexpr int NL = HalfLines<M>::value;
     __shared__ T2 buf[NL * LdsLine<M>::stride];
     __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
@@ -938,7 +949,7 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int rows = g.gz * g.gy;
-            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, Lines<M>::value), nj),
+            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, HalfLines<M>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
                                tw + twiddle_offset(2 * M), rows, g.gy, g.Dy);
         }
@@ -948,7 +959,7 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int rows = g.gz * g.gy;
-            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, Lines<M>::value), nj),
+            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, HalfLines<M>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
                                tw + twiddle_offset(2 * M), rows, g.gy, g.Dy);
         }
