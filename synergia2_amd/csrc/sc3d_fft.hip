@@ -752,10 +752,9 @@ template <typename T> __device__ __forceinline__ T* green_buffer(const GridJob& 
 // G1: rows (Z<=gz, Y<=gy) of the doubled grid: generate, R2C along x, keep
 // the real part of kx = 0..M (the row is even, its spectrum real).
 template <typename T, int M>
-__global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict__ jobs,
-                                                      const typename cplx<T>::type* __restrict__ twM,
-                                                      const typename cplx<T>::type* __restrict__ tw2M,
-                                                      int gy, int gz, int green_fn)
+__global__ __launch_bounds__(kThreads) void k_green_x(
+    const GridJob* __restrict__ jobs, const typename cplx<T>::type* __restrict__ twM,
+    const typename cplx<T>::type* __restrict__ tw2M, int gy, int gz, int green_fn)
 {
     using T2 = typename cplx<T>::type;
     constexpr int NL = Lines<M>::value;
@@ -832,9 +831,9 @@ __global__ __launch_bounds__(kThreads) void k_green_x(const GridJob* __restrict_
 // AXIS 1 (y): address (plane*(gy+1) + p)*Hx + kx, plane = Z <= gz.
 // AXIS 2 (z): address (p*(gy+1) + plane)*Hx + kx, plane = ky <= gy.
 template <typename T, int L, int AXIS>
-__global__ __launch_bounds__(kThreads) void k_green_cols(const GridJob* __restrict__ jobs,
-                                                         const typename cplx<T>::type* __restrict__ tw,
-                                                         int ncols, int gy, int Hx)
+__global__ __launch_bounds__(kThreads) void k_green_cols(
+    const GridJob* __restrict__ jobs, const typename cplx<T>::type* __restrict__ tw, int ncols,
+    int gy, int Hx)
 {
     using T2 = typename cplx<T>::type;
     constexpr int NL = GreenLines<L, T2>::value;

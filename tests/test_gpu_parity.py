@@ -115,6 +115,14 @@ def test_green_spectrum_matches_full_fft(gpu):
         got = op.debug(sc.DBG_GHAT).reshape(gz + 1, gy + 1, hx)
         assert rel_l2(got, ref) <= 1e-13, gname
         assert np.abs(full.imag).max() <= 1e-12 * np.abs(full.real).max()
+        # precision form: the same FP64 values, transformed in FP32 (measured
+        # 9e-8 for both functions; a wrong twiddle or index is an O(1) error)
+        op32, _ = gpu_run(make_opts(shape, green, precision=sc.FP32_GRID), parts, masks, 1e10,
+                          1e-9)
+        got32 = op32.debug(sc.DBG_GHAT).reshape(gz + 1, gy + 1, hx)
+        err32 = rel_l2(got32, ref)
+        print(f"  FP32 spectrum rel-L2 {err32:.2e}", flush=True)
+        assert err32 <= 1e-6, gname
 
 
 @pytest.mark.parametrize("case", ["pointlike_fixed_masked", "anisotropic_outliers"])
