@@ -55,14 +55,21 @@ template <int L> struct Lines {
 };
 
 // Half as many lines (>= 4, >= 512 elements) where more, smaller blocks pay:
-// the solve's x passes (rows are contiguous, so the line count does not set
-// the run width) and the Green's column passes.  These launches hold a few
-// blocks per CU at 64x64x128 and are latency-bound.  Measured (us per step,
-// 64x64x128 FP64 / 128x128x256 FP32): solve 59.4 -> 58.5 / 273 -> 265;
-// Green's function 35.5 -> 34.9 / 112 -> 111.  k_green_x keeps Lines (its
-// FP32 pass at 128x128x256: Green's phase 112 -> 119 us with half).
+// the x passes of the solve and of the field (rows are contiguous, so the
+// line count does not set the run width) and the Green's column passes.
+// These launches are latency-bound and at 64x64x128 held two to four blocks
+// per CU.  Measured (us per kernel): at 64x64x128 FP64, x R2C 6.9 -> 5.8,
+// x C2R 5.6 -> 4.9, the two Green's column passes 17.5 -> 16.3; at
+// 128x128x256 FP32, the field x pass 50.4 -> 43.3.  k_green_x keeps Lines
+// (its FP32 pass at 128x128x256: Green's phase 112 -> 119 us with half).
 template <int L> struct HalfLines {
     static constexpr int value = (512 / L) > 4 ? (512 / L) : 4;
+};
+// The solve's x R2C / C2R passes: half only for rows shorter than 128 (the
+// grids with few rows; at 128x128x256 the R2C pass is 18.5 us with Lines,
+// 21.6 with half, its 32768 rows giving 4096 blocks already).
+template <int M> struct XLines {
+    static constexpr int value = M >= 128 ? Lines<M>::value : HalfLines<M>::value;
 };
 
 // Column passes: enough lines per block that each strided load or store
@@ -262,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_x_r2c(const GridJob* __restrict__ 
                                                     int nrows, int gy, int Dy)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = HalfLines<M>::value;
+    constexpr int NL = XLines<M>::value;
     __shared__ T2 buf[NL * LdsLine<M>::stride];
     __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void k_x_c2r(const GridJob* __restrict__ 
                                                     int nrows, int gy, int Dy)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = HalfLines<M>::value;
+    constexpr int NL = XLines<M>::value;
     __shared__ T2 buf[NL * LdsLine<M>::stride];
     __shared__ T2 twl[M];
     const GridJob& J = jobs[blockIdx.y];
@@ -948,7 +955,7 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int rows = g.gz * g.gy;
-            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, HalfLines<M>::value), nj),
+            hipLaunchKernelGGL((k_x_r2c<R, M>), dim3(nblocks(rows, XLines<M>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
                                tw + twiddle_offset(2 * M), rows, g.gy, g.Dy);
         }
@@ -958,7 +965,7 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int rows = g.gz * g.gy;
-            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, HalfLines<M>::value), nj),
+            hipLaunchKernelGGL((k_x_c2r<R, M>), dim3(nblocks(rows, XLines<M>::value), nj),
                                dim3(kThreads), 0, st, jobs, tw + twiddle_offset(M),
                                tw + twiddle_offset(2 * M), rows, g.gy, g.Dy);
         }
