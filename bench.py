@@ -114,8 +114,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the step would "
+              "shard by particles, with one all-reduce of the density octant per step)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
