@@ -85,7 +85,8 @@ template <int L, typename T2> struct ColLines {
 // 160 KiB of LDS per CU.
 // (Measured at 64x64x128 and 128x128x256: 32 KiB for FP64 lines (24 KiB
 // the same, 48 KiB slower); 32 KiB for FP32 lines (field passes 273 us vs
-// 280 us at 48 KiB, 303 us at 64 KiB).)
+// 280 us at 48 KiB, 303 us at 64 KiB).  16 KiB for both: solve 57 -> 62 us
+// at 64x64x128, 265 -> 291 us at 128x128x256.)
 #ifndef SC3D_LDS_CAP_FP64
 #define SC3D_LDS_CAP_FP64 32768
 #endif
