@@ -710,6 +710,32 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
     e = (a - (s - bb)) + (b - bb);
 }
 
+// Lane moves inside the register file (DPP, GFX9 controls) for the
+// wavefront scans of the deposit, instead of LDS permutes: rows of 16 lanes;
+// a lane with no source lane, or in a row outside ROW_MASK, returns `old`.
+constexpr int kDppRowShr = 0x110;  // + n: from lane - n of the same row
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;  // from lane + 1 / lane - 1
+constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
+constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_move(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_move(float old, float v)
+{
+    return __int_as_float(dpp_move<CTRL, ROW_MASK>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double old, double v)
+{
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const unsigned lo = unsigned(dpp_move<CTRL, ROW_MASK>(int(o), int(x)));
+    const int hi = dpp_move<CTRL, ROW_MASK>(int(o >> 32), int(x >> 32));
+    return __longlong_as_double((long long)(uint64_t(lo) | (uint64_t(unsigned(hi)) << 32)));
+}
+
 // Eight corner sums: FP64 plain, FP32 compensated.
 template <typename R> struct CornerSums;
 template <> struct CornerSums<double> {
@@ -719,11 +745,11 @@ template <> struct CornerSums<double> {
 #pragma unroll
         for (int c = 0; c < 8; ++c) w[c] += o.w[c];
     }
-    __device__ __forceinline__ CornerSums shfl_up(int s) const
+    template <int CTRL, int ROW_MASK = 0xf> __device__ __forceinline__ CornerSums moved() const
     {
         CornerSums o;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) o.w[c] = __shfl_up(w[c], s, 64);
+        for (int c = 0; c < 8; ++c) o.w[c] = dpp_move<CTRL, ROW_MASK>(0.0, w[c]);
         return o;
     }
     __device__ __forceinline__ double total(int c) const { return w[c]; }
@@ -740,13 +766,13 @@ template <> struct CornerSums<float> {
             e[c] = (e[c] + o.e[c]) + err;
         }
     }
-    __device__ __forceinline__ CornerSums shfl_up(int s) const
+    template <int CTRL, int ROW_MASK = 0xf> __device__ __forceinline__ CornerSums moved() const
     {
         CornerSums o;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            o.w[c] = __shfl_up(w[c], s, 64);
-            o.e[c] = __shfl_up(e[c], s, 64);
+            o.w[c] = dpp_move<CTRL, ROW_MASK>(0.0f, w[c]);
+            o.e[c] = dpp_move<CTRL, ROW_MASK>(0.0f, e[c]);
         }
         return o;
     }
@@ -821,6 +847,16 @@ template <typename R> struct AtomicRuns {
     }
 };
 
+// One step of the segmented scan: add the partial sum the lane move brings
+// when it comes from the same segment.
+template <int CTRL, int ROW_MASK, typename R>
+__device__ __forceinline__ void scan_step(CornerSums<R>& cur, int seg)
+{
+    const int seg_from = dpp_move<CTRL, ROW_MASK>(-1, seg);
+    const CornerSums<R> up = cur.template moved<CTRL, ROW_MASK>();
+    if (seg_from == seg) cur.add(up);
+}
+
 template <typename R, int K, typename Emit>
 __device__ __forceinline__ void deposit_runs(const int (&key)[K], const double (&ox)[K],
                                              const double (&oy)[K], const double (&oz)[K],
@@ -857,21 +893,25 @@ __device__ __forceinline__ void deposit_runs(const int (&key)[K], const double (
     }
     if (single) head_key = cur_key;
     // trailing runs across lanes: segmented inclusive scan (every lane takes
-    // part in every shuffle: a lane shuffle must never run under a divergent
-    // exec mask)
-    const int prev_tail = __shfl_up(cur_key, 1, 64);
+    // part in every lane move: a move must never run under a divergent exec
+    // mask).  seg >= 1 numbers the segments; a lane adds a moved partial sum
+    // only when it comes from its own segment (-1 marks no source lane).
+    const int prev_tail = dpp_move<kDppWaveShr1>(0, cur_key);
     const bool start = lane == 0 || !single || prev_tail != cur_key;
     const int seg = __popcll(__ballot(start) & le_mask);
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const int seg_up = __shfl_up(seg, s, 64);
-        const CornerSums<R> up = cur.shfl_up(s);
-        if ((lane >= s) & (seg_up == seg)) cur.add(up);
-    }
+    // within rows of 16 lanes: from lane - 1, - 2, - 4, - 8 of the row
+    scan_step<kDppRowShr + 1, 0xf>(cur, seg);
+    scan_step<kDppRowShr + 2, 0xf>(cur, seg);
+    scan_step<kDppRowShr + 4, 0xf>(cur, seg);
+    scan_step<kDppRowShr + 8, 0xf>(cur, seg);
+    // across rows: lane 15 into row 1 and lane 47 into row 3, then lane 31
+    // (by now the sum through lane 31) into rows 2 and 3
+    scan_step<kDppRowBcast15, 0xa>(cur, seg);
+    scan_step<kDppRowBcast31, 0xc>(cur, seg);
     // the total of the previous lane's trailing run continues this lane's
     // leading run when the keys agree
-    const CornerSums<R> carry = cur.shfl_up(1);
-    const int next_head = __shfl_down(head_key, 1, 64);
+    const CornerSums<R> carry = cur.template moved<kDppWaveShr1>();
+    const int next_head = dpp_move<kDppWaveShl1>(0, head_key);
     if (!single && head_key >= 0) {
         if (lane > 0 && prev_tail == head_key) head.add(carry);
         emit(head_key, head, head_last);
