@@ -419,19 +419,48 @@ __device__ __forceinline__ int64_t count_at(int d, int64_t b, int nblk)
     return int64_t(d) * nblk + b;
 }
 
-// Exclusive prefix of one value per thread over the block (wavefront
-// shuffles, one LDS exchange of the wavefront sums).  `wsum` holds
-// kThreads/64 entries; callers separate two uses by a barrier.
+// Lane moves inside the register file (DPP, GFX9 controls) for the
+// wavefront scans, instead of LDS permutes: rows of 16 lanes; a lane with no
+// source lane, or in a row outside ROW_MASK, returns `old`.
+constexpr int kDppRowShr = 0x110;  // + n: from lane - n of the same row
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;  // from lane + 1 / lane - 1
+constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
+constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_move(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_move(float old, float v)
+{
+    return __int_as_float(dpp_move<CTRL, ROW_MASK>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double old, double v)
+{
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const unsigned lo = unsigned(dpp_move<CTRL, ROW_MASK>(int(o), int(x)));
+    const int hi = dpp_move<CTRL, ROW_MASK>(int(o >> 32), int(x >> 32));
+    return __longlong_as_double((long long)(uint64_t(lo) | (uint64_t(unsigned(hi)) << 32)));
+}
+
+// Exclusive prefix of one value per thread over the block (DPP lane moves
+// inside each wavefront, one LDS exchange of the wavefront sums); every
+// thread of the block calls it.  `wsum` holds kThreads/64 entries; callers
+// separate two uses by a barrier.
 __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* wsum, int32_t& total)
 {
     constexpr int kWaves = kThreads / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // inclusive wavefront scan: inside rows of 16 lanes, then across rows
     int32_t x = v;
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const int32_t y = __shfl_up(x, s, 64);
-        if (lane >= s) x += y;
-    }
+    x += dpp_move<kDppRowShr + 1>(0, x);
+    x += dpp_move<kDppRowShr + 2>(0, x);
+    x += dpp_move<kDppRowShr + 4>(0, x);
+    x += dpp_move<kDppRowShr + 8>(0, x);
+    x += dpp_move<kDppRowBcast15, 0xa>(0, x);
+    x += dpp_move<kDppRowBcast31, 0xc>(0, x);
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
     int32_t off = 0, tot = 0;
@@ -708,32 +737,6 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
     s = a + b;
     const float bb = s - a;
     e = (a - (s - bb)) + (b - bb);
-}
-
-// Lane moves inside the register file (DPP, GFX9 controls) for the
-// wavefront scans of the deposit, instead of LDS permutes: rows of 16 lanes;
-// a lane with no source lane, or in a row outside ROW_MASK, returns `old`.
-constexpr int kDppRowShr = 0x110;  // + n: from lane - n of the same row
-constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;  // from lane + 1 / lane - 1
-constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
-constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ int dpp_move(int old, int v)
-{
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
-}
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ float dpp_move(float old, float v)
-{
-    return __int_as_float(dpp_move<CTRL, ROW_MASK>(__float_as_int(old), __float_as_int(v)));
-}
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ double dpp_move(double old, double v)
-{
-    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
-    const unsigned lo = unsigned(dpp_move<CTRL, ROW_MASK>(int(o), int(x)));
-    const int hi = dpp_move<CTRL, ROW_MASK>(int(o >> 32), int(x >> 32));
-    return __longlong_as_double((long long)(uint64_t(lo) | (uint64_t(unsigned(hi)) << 32)));
 }
 
 // Eight corner sums: FP64 plain, FP32 compensated.
