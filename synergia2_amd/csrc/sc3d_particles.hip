@@ -1038,6 +1038,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(float) ? 5 : 1))) void k_
     __shared__ int32_t wave_off[kWaves][kMaxBuckets];
     __shared__ int32_t start_d[kMaxBuckets];
     __shared__ int32_t wsum[kWaves];
+    __shared__ int32_t edge_run[kWaves];
     if (dom->error) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
@@ -1184,9 +1185,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(float) ? 5 : 1))) void k_
     int32_t nruns;
     const int32_t base_run = block_exclusive_scan(ends, wsum, nruns);
     const bool listed = nruns <= kRunCap;
+    // A run is emitted once per wavefront that holds part of it (the lane
+    // scan joins parts inside a wavefront only), so only a run across a
+    // wavefront edge has several emitters: the run holding wavefront w's
+    // first position when the position before has the same key.  Its list
+    // entry is cleared and summed with atomics; every other entry is written
+    // once with plain stores.
+    if (lane == 0)
+        edge_run[wave] =
+            wave > 0 && key[0] != sentinel && skey[cur][4 * t - 1] == key[0] ? base_run : -1;
     __syncthreads();  // the sort buffers are read: the run list may overwrite them
-    if (listed)
-        for (int e = t; e < nruns * 8; e += kThreads) rw[e >> 3][e & 7] = R(0);
+    if (listed && t < kWaves && edge_run[t] >= 0)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) rw[edge_run[t]][c] = R(0);
     __syncthreads();
     double ox[4], oy[4], oz[4];
 #pragma unroll
@@ -1208,11 +1219,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(float) ? 5 : 1))) void k_
         return;
     }
     // runs into the list (a run cut by a wavefront edge arrives in parts)
+    static_assert(kWaves == 4, "edge runs of three wavefront edges");
+    const int e1 = edge_run[1], e2 = edge_run[2], e3 = edge_run[3];
     auto to_list = [&](int k, const CornerSums<R>& a, int q_last) {
         const int r = base_run + before[q_last];
         rkey[r] = k;
+        if (r == e1 || r == e2 || r == e3) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) atomicAdd(&rw[r][c], R(a.total(c)));
+            for (int c = 0; c < 8; ++c) atomicAdd(&rw[r][c], R(a.total(c)));
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) rw[r][c] = R(a.total(c));
+        }
     };
     deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, to_list);
     __syncthreads();
