@@ -1000,32 +1000,35 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // window (key order, over the sort buffers); there each cell adds its four
 // +x corners together with the -x corners of the next cell when that cell
 // is its row neighbour, so shared nodes take one atomic instead of two (a
-// window with more than kRunCap runs adds its runs directly).  A cell whose
+// window with more runs than the list holds adds its runs directly).  A cell whose
 // particles fall in several windows is added once per window.  Measured at
 // 4e6 / 1.6e7 particles (deposit phase, us, sort_period 8, before the run
 // list): windows of 1024 with the offsets held in LDS 158 / 590; 1024
 // re-reading the coordinates 141 / 493; 2048 re-reading 137 / 530; 4096
 // re-reading 166 / 726.  The run list: 104 / 455.
 constexpr int kWindow = 1024;
-#ifndef SC3D_RUN_CAP
-#define SC3D_RUN_CAP 512
-#endif
-constexpr int kRunCap = SC3D_RUN_CAP;  // runs per window held in LDS (more: direct atomics)
 
-// The kernel is latency-bound, so blocks per CU matter.  FP64: 40 KiB of LDS
-// and 118 VGPRs, both limits at 4 blocks per CU.  FP32: 24 KiB of LDS, so the
-// registers are held to 96 (5 waves per SIMD, a few values spilled): at 1.6e7
-// particles 405 us against 448 us at 109 VGPRs (4 waves).  Measured and not
-// kept: 6 waves (80 VGPRs, more spills) 477 us for FP32 and 125 us against 109
-// for FP64 at 4e6; 5 waves for FP64 (run list cut to 384 to fit the LDS)
-// 119 us.
+// The kernel is latency-bound, so blocks per CU matter; both the run list
+// (LDS) and the registers are sized for them.  FP64: a list of 384 runs
+// (31 KiB of LDS) and 96 VGPRs (8 bytes spilled) give 5 blocks per CU; FP32:
+// 512 runs (24 KiB) and 80 VGPRs (76 bytes spilled) give 6.  Measured (us per
+// deposit, 4e6 FP64 / 1.6e7 FP32) against 512 runs and 4 / 5 waves per SIMD:
+// 96.7 vs 100.7 / 334 vs 359; FP32 with the 384-run list 364.  (Before the
+// run list's plain stores, 5 FP64 and 6 FP32 waves measured slower: their
+// spills were larger.)
+template <typename R> struct DepositShape {
+    static constexpr bool fp64 = sizeof(R) == sizeof(double);
+    static constexpr int run_cap = fp64 ? 384 : 512;  // runs held in LDS (more: direct atomics)
+    static constexpr int waves = fp64 ? 5 : 6;        // per SIMD
+};
 template <typename R>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(float) ? 5 : 1))) void k_deposit_local(
+__attribute__((amdgpu_waves_per_eu(DepositShape<R>::waves))) void k_deposit_local(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
     int64_t n, int gx, int gy, int gz, const DomainRec* __restrict__ dom, R* __restrict__ rho)
 {
     constexpr int W = kWindow, PER = kWindow / kThreads, kWaves = kThreads / 64;
+    constexpr int kRunCap = DepositShape<R>::run_cap;
     // the sort buffers and, after the sort, the window's run list share LDS
     constexpr int kSortBytes = 4 * W * int(sizeof(int32_t));
     constexpr int kListBytes = kRunCap * int(sizeof(int32_t) + 8 * sizeof(R));
