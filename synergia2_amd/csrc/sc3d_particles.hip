@@ -996,11 +996,13 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // in LDS (stable LSD radix over the bits of key - min key of the window,
 // <= 8-bit digits, wave-private ranking as in the global scatter; skipped
 // when the window is already in cell order), and sums it in that order with
-// the lane runs (4 particles per lane).  The runs go into an LDS list of the
-// window (key order, over the sort buffers); there each cell adds its four
-// +x corners together with the -x corners of the next cell when that cell
-// is its row neighbour, so shared nodes take one atomic instead of two (a
-// window with more runs than the list holds adds its runs directly).  A cell whose
+// the lane runs (4 particles per lane; the wavefront scan moves partial sums
+// between lanes with DPP).  The runs go into an LDS list of the window (key
+// order, over the sort buffers; plain stores, atomics only for the runs
+// across a wavefront edge); there each cell adds its four +x corners
+// together with the -x corners of the next cell when that cell is its row
+// neighbour, so shared nodes take one atomic instead of two (a window with
+// more runs than the list holds adds its runs directly).  A cell whose
 // particles fall in several windows is added once per window.  Measured at
 // 4e6 / 1.6e7 particles (deposit phase, us, sort_period 8, before the run
 // list): windows of 1024 with the offsets held in LDS 158 / 590; 1024
