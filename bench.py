@@ -10,8 +10,9 @@ function, FP64 throughout.
 
 A step = one space-charge step (the hot path: bunch extent, cell sort and
 reorder of the bunch every sort_period steps, deposit, Green's spectrum, FFT
-convolution, E field, kick) of a bunch that is tracked through the channel: the half-step drift/quadrupole maps run between
-the kicks as their own kernels.  value = macroparticles x steps / (sum of the
+convolution, E field, kick) of a bunch that is tracked through the channel:
+the half-step drift/quadrupole maps run between the kicks as their own
+kernels.  value = macroparticles x steps / (sum of the
 HIP-event times of the timed steps, first kernel of the step to last, on the
 stream they run on) = particle-kicks/s.  The wall time of the whole timed
 loop, maps included, is reported beside it as the tracking rate.
