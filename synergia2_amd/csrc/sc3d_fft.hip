@@ -491,7 +491,8 @@ __device__ __forceinline__ auto sin_2pi(const T2* __restrict__ tw, int k, int L)
 // forward transform so that both global loads are in flight together.
 // Lines per block: at most SC3D_ZF_LINES (measured at 128x128x256: 4 lines
 // 287 us, 8 lines 323 us, 16 lines 424 us for the FP32 field passes; the
-// pass is latency-bound and more lines cost occupancy).
+// pass is latency-bound and more lines cost occupancy; later, with the
+// current passes, 2 lines 287 us against 266 with 4: 16-byte runs).
 #ifndef SC3D_ZF_LINES
 #define SC3D_ZF_LINES 4
 #endif
