@@ -334,6 +334,21 @@ def test_green_cache_fixed_domain(gpu):
     assert np.array_equal(op.debug(sc.DBG_GHAT), g1)
     # positions unchanged by the kick -> same rho, same phi up to atomics order
     assert rel_l2(op.debug(sc.DBG_PHI), phi1) <= 1e-14
+    # precision form: its FP32 spectrum is cached the same way, and the field
+    # of the second apply matches the first (same positions)
+    op32 = sc.SpaceCharge3dOpenHockney(make_opts((32, 32, 64), fixed=fixed,
+                                                 precision=sc.FP32_GRID))
+    b32 = sc.Bunch(n, 1e10, gamma=GAMMA)
+    b32.set_host(parts, masks)
+    op32.apply_bunch(b32, 1e-9)
+    assert op32.debug(sc.DBG_DOMAIN)["ghat_dirty"] == 1
+    g32 = op32.debug(sc.DBG_GHAT)
+    e32 = op32.debug(sc.DBG_EX)
+    assert rel_l2(g32, g1) <= 1e-6
+    op32.apply_bunch(b32, 1e-9)
+    assert op32.debug(sc.DBG_DOMAIN)["ghat_dirty"] == 0
+    assert np.array_equal(op32.debug(sc.DBG_GHAT), g32)
+    assert rel_l2(op32.debug(sc.DBG_EX), e32) <= 1e-6
 
 
 def test_errors_and_empty(gpu):
