@@ -83,6 +83,7 @@ typedef struct sc3d_server_options {
 #define SC3D_JOB_WAITING 0
 #define SC3D_JOB_RESIDENT 1
 #define SC3D_JOB_DONE 2
+#define SC3D_JOB_FAILED 3             /* admission failed (allocation); not retried */
 
 typedef struct sc3d_job_result {
     int64_t id;

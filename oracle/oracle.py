@@ -126,6 +126,13 @@ def lib():
         L.orc_reference_cdt.restype = C.c_double
         L.orc_apply_maps.argtypes = [C.POINTER(Bunch), C.POINTER(Slice), C.c_int]
         L.orc_moments.argtypes = [dp, C.c_int64, C.c_int64, C.POINTER(C.c_uint8), C.c_double, dp, dp]
+        u32p = C.POINTER(C.c_uint32)
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_det_log.argtypes = [C.c_double]
+        L.orc_det_log.restype = C.c_double
+        L.orc_det_sincos2pi.argtypes = [C.c_double, dp, dp]
+        L.orc_populate_gaussian.argtypes = [dp, C.c_int64, C.c_int64, C.POINTER(C.c_uint8), dp, dp,
+                                            C.c_uint64, C.c_int64]
         _lib = L
     return _lib
 
@@ -298,3 +305,34 @@ def moments(parts, masks, total_num=None):
     lib().orc_moments(_dp(parts), parts.shape[1], n, masks.ctypes.data_as(C.POINTER(C.c_uint8)),
                       float(total_num if total_num is not None else n), _dp(mean), _dp(mom2))
     return mean, mom2.reshape(6, 6)
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, out)
+    return [int(x) for x in out]
+
+
+def det_log(u):
+    return lib().orc_det_log(float(u))
+
+
+def det_sincos2pi(u):
+    s, c = C.c_double(), C.c_double()
+    lib().orc_det_sincos2pi(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def populate_gaussian(n, mean, chol, seed, id0=0, capacity=None):
+    """(parts (7, capacity), masks (n,)) of the counter-based Gaussian
+    populate: chol is the packed lower-triangular factor (21 entries)."""
+    cap = capacity or n
+    parts = np.zeros((7, cap))
+    masks = np.zeros(n, np.uint8)
+    lib().orc_populate_gaussian(_dp(parts), cap, n, masks.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                _dp(np.ascontiguousarray(mean, np.float64)),
+                                _dp(np.ascontiguousarray(chol, np.float64)), C.c_uint64(seed),
+                                int(id0))
+    return parts, masks

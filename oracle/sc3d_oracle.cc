@@ -23,7 +23,14 @@
 //                           conservation, brute-force O(8N) numpy deposit.
 //   convolution (apply)  -- brute-force direct sum on a tiny grid.
 //   orc_force            -- linear/quadratic phi gives exact differences.
-//   orc_kick             -- per-particle closed form; rod-of-charge physics.
+//   orc_kick             -- rod-of-charge physics on the x and y axes;
+//                           Gaussian-sphere probes on each axis (closed-form
+//                           field: dp/p, x', y' and no cross terms).
+//   orc_moments          -- numpy mean / centred cov, exact Hadamard
+//                           fixture, restated summation tree (bit-equal).
+//   orc_philox4x32_10    -- Random123 known-answer vectors; numpy restatement.
+//   orc_det_log/sincos2pi-- 200-bit arithmetic (mpmath), within 1 / 2 ulp.
+//   orc_populate_gaussian-- the pieces above; sample moments vs L L^T.
 #include "sc3d_oracle.h"
 
 #include <algorithm>
@@ -987,4 +994,129 @@ extern "C" void orc_moments(const double* p, int64_t st, int64_t n, const uint8_
             mom2[j * 6 + k] = mom2[k * 6 + j] = s2[q] / total_num;
             ++q;
         }
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian populate of the test bunches (the reference draws with GSL's
+// ranlxd2 and gsl_ran_gaussian in populate_6d, populate.cc:128-145, then
+// applies the Cholesky factor of the covariance; the build replaces the
+// generator by a counter-based one so that host and device produce the same
+// bunch).  Published algorithms restated here:
+//   Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+//   easy as 1, 2, 3", SC'11): round (c0..c3) -> (hi(M1 c2) ^ c1 ^ k0,
+//   lo(M1 c2), hi(M0 c0) ^ c3 ^ k1, lo(M0 c0)), M0 = 0xD2511F53,
+//   M1 = 0xCD9E8D57; key bumped by (0x9E3779B9, 0xBB67AE85) between rounds.
+//   Counter (id lo, id hi, call, 0), key (seed lo, seed hi), calls 0..2.
+//   Uniform: the top 53 bits m of (r0, r1) or (r2, r3), (m + 0.5) 2^-53.
+//   Box-Muller (1958): g = sqrt(-2 log u1) (cos 2 pi u2, sin 2 pi u2), with
+//   log and sin/cos evaluated from IEEE basic operations only:
+//     log: u = m 2^e, m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f),
+//          log(1+f) = f - (f^2/2 - s (f^2/2 + R)), R = sum_{k=1..11}
+//          2 s^2k/(2k+1) (the arrangement of fdlibm's e_log.c), plus e ln 2
+//          with ln 2 split into a 32-bit head and its tail;
+//     sin/cos: quadrant of 4u, angle folded to [0, pi/4], Taylor series to
+//          x^19 / x^20, exact quadrant rotation.
+//   Coordinates: x_r = mean_r + sum_{c<=r} L_rc g_c in that order (L packed
+//   lower-triangular, row-major); column 6 = id0 + slot.
+// ---------------------------------------------------------------------------
+extern "C" void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = uint32_t(p1);
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = uint32_t(p0);
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+extern "C" double orc_det_log(double u)
+{
+    const double ln2_head = 0x1.62e42feep-1;
+    const double ln2_tail = 0x1.a39ef35793c76p-33;
+    int e = 0;
+    double m = std::frexp(u, &e);
+    if (m < 0x1.6a09e667f3bcdp-1) {  // sqrt(1/2)
+        m = m + m;
+        e = e - 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double R = 2.0 / 23.0;
+    for (int k = 10; k >= 1; --k) R = 2.0 / double(2 * k + 1) + z * R;
+    R = z * R;
+    const double hfsq = 0.5 * f * f;
+    const double de = double(e);
+    return de * ln2_head + (f - (hfsq - (s * (hfsq + R) + de * ln2_tail)));
+}
+
+extern "C" void orc_det_sincos2pi(double u, double* sn, double* cs)
+{
+    const double half_pi = 0x1.921fb54442d18p+0;
+    const double t = 4.0 * u;
+    const double q = std::floor(t);
+    const double r = t - q;
+    const bool fold = r > 0.5;
+    const double x = (fold ? 1.0 - r : r) * half_pi;
+    const double z = x * x;
+    // (2k+1)! and (2k)! are exact doubles up to 20!
+    double fact[21];
+    fact[0] = 1.0;
+    for (int k = 1; k <= 20; ++k) fact[k] = fact[k - 1] * double(k);
+    double S = -1.0 / fact[19];
+    for (int k = 8; k >= 1; --k) S = ((k % 2) ? -1.0 : 1.0) / fact[2 * k + 1] + z * S;
+    const double sin_x = x + x * (z * S);
+    double Cq = 1.0 / fact[20];
+    for (int k = 9; k >= 1; --k) Cq = ((k % 2) ? -1.0 : 1.0) / fact[2 * k] + z * Cq;
+    const double cos_x = 1.0 + z * Cq;
+    const double sa = fold ? cos_x : sin_x, ca = fold ? sin_x : cos_x;
+    switch (int(q) & 3) {
+    case 0: *sn = sa; *cs = ca; break;
+    case 1: *sn = ca; *cs = -sa; break;
+    case 2: *sn = -sa; *cs = -ca; break;
+    default: *sn = -ca; *cs = sa; break;
+    }
+}
+
+extern "C" void orc_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,
+                                      const double mean[6], const double chol[21],
+                                      uint64_t seed, int64_t id0)
+{
+    const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t id = uint64_t(id0 + i);
+        double u[6];
+        for (uint32_t call = 0; call < 3; ++call) {
+            const uint32_t ctr[4] = {uint32_t(id), uint32_t(id >> 32), call, 0u};
+            uint32_t r[4];
+            orc_philox4x32_10(ctr, key, r);
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t m = (uint64_t(r[2 * h]) << 21) | (r[2 * h + 1] >> 11);
+                u[2 * call + h] = (double(m) + 0.5) * 0x1.0p-53;
+            }
+        }
+        double g[6];
+        for (int k = 0; k < 3; ++k) {
+            const double rad = std::sqrt(-2.0 * orc_det_log(u[2 * k]));
+            double sn, cs;
+            orc_det_sincos2pi(u[2 * k + 1], &sn, &cs);
+            g[2 * k] = rad * cs;
+            g[2 * k + 1] = rad * sn;
+        }
+        for (int r = 0; r < 6; ++r) {
+            double v = mean[r];
+            for (int c = 0; c <= r; ++c) v += chol[r * (r + 1) / 2 + c] * g[c];
+            parts[i + r * stride] = v;
+        }
+        parts[i + 6 * stride] = double(id0 + i);
+        masks[i] = 1;
+    }
 }

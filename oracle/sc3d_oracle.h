@@ -126,6 +126,14 @@ void orc_moments(const double* parts, int64_t stride, int64_t n, const uint8_t* 
                  double total_num, double mean[6], double mom2[36]);
 void orc_apply_maps(orc_bunch* b, const orc_slice* s, int nslices);
 
+/* Counter-based Gaussian populate (the build's generator; see the .cc). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_det_log(double u);
+void orc_det_sincos2pi(double u, double* sn, double* cs);
+void orc_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,
+                           const double mean[6], const double chol[21], uint64_t seed,
+                           int64_t id0);
+
 #ifdef __cplusplus
 }
 #endif

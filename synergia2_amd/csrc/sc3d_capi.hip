@@ -9,6 +9,7 @@
 // Everything is enqueued on the caller's stream; no host synchronisation
 // happens inside sc3d_apply (the domain is finished on the device).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -24,26 +25,41 @@ bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1))
 
 #define HIP_TRY SC3D_HIP_TRY
 
+// Releases the particle-side buffers and forgets them: no pointer to freed
+// memory and no stale capacity survive, so a failed reallocation leaves a
+// context that simply reallocates on its next apply.
+void drop_particle_buffers(sc3d_ctx* c)
+{
+    for (void** p : {(void**)&c->sb.keys0, (void**)&c->sb.keysA, (void**)&c->sb.keysB,
+                     (void**)&c->sb.valsA, (void**)&c->sb.valsB, (void**)&c->sb.counts,
+                     (void**)&c->sb.totals, (void**)&c->scratch, (void**)&c->smask}) {
+        dfree(c, *p);
+        *p = nullptr;
+    }
+    c->sb.perm = nullptr;
+    c->pcap = 0;
+    c->sorted_n = -1;
+}
+
 int ensure_particle_buffers(sc3d_ctx* c, int64_t n)
 {
     if (n <= c->pcap) return 0;
-    int64_t cap = n + n / 8 + 4096;
-    for (void* p : {(void*)c->sb.keys0, (void*)c->sb.keysA, (void*)c->sb.keysB,
-                    (void*)c->sb.valsA, (void*)c->sb.valsB, (void*)c->sb.counts,
-                    (void*)c->sb.totals, (void*)c->scratch, (void*)c->smask})
-        dfree(c, p);
-    int64_t tiles = (cap + kSortTileItems - 1) / kSortTileItems;
-    int rc = 0;
-    rc |= dmalloc(c, &c->sb.keys0, cap);
-    rc |= dmalloc(c, &c->sb.keysA, cap);
-    rc |= dmalloc(c, &c->sb.keysB, cap);
-    rc |= dmalloc(c, &c->sb.valsA, cap);
-    rc |= dmalloc(c, &c->sb.valsB, cap);
-    rc |= dmalloc(c, &c->sb.counts, 2048 * tiles);
-    rc |= dmalloc(c, &c->sb.totals, 2048);
-    rc |= dmalloc(c, &c->scratch, 7 * cap);
-    rc |= dmalloc(c, &c->smask, cap);
-    if (rc) return rc;
+    drop_particle_buffers(c);
+    const int64_t cap = n + n / 8 + 4096;
+    const int64_t tiles = (cap + kSortTileItems - 1) / kSortTileItems;
+    int rc = dmalloc(c, &c->sb.keys0, cap);
+    if (!rc) rc = dmalloc(c, &c->sb.keysA, cap);
+    if (!rc) rc = dmalloc(c, &c->sb.keysB, cap);
+    if (!rc) rc = dmalloc(c, &c->sb.valsA, cap);
+    if (!rc) rc = dmalloc(c, &c->sb.valsB, cap);
+    if (!rc) rc = dmalloc(c, &c->sb.counts, 2048 * tiles);
+    if (!rc) rc = dmalloc(c, &c->sb.totals, 2048);
+    if (!rc) rc = dmalloc(c, &c->scratch, 7 * cap);
+    if (!rc) rc = dmalloc(c, &c->smask, cap);
+    if (rc) {
+        drop_particle_buffers(c);
+        return rc;
+    }
     c->pcap = cap;
     return 0;
 }
@@ -87,6 +103,13 @@ int fail(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
+}
+
+int64_t test_alloc_limit()
+{
+    const char* v = std::getenv("SC3D_TEST_MAX_ALLOC_BYTES");
+    if (!v || !*v) return -1;
+    return std::strtoll(v, nullptr, 10);
 }
 
 void dfree(sc3d_ctx* c, void* p)
@@ -151,6 +174,8 @@ int prepare_apply(sc3d_ctx* c, const sc3d_bunch* b)
     const int64_t nch = (n + 4095) / 4096;
     if (!c->opt.domain_fixed && nch * 6 > c->P_cap) {
         dfree(c, c->P);
+        c->P = nullptr;
+        c->P_cap = 0;
         if (int rc = dmalloc(c, &c->P, nch * 6 + 64)) return rc;
         c->P_cap = nch * 6 + 64;
     }
@@ -549,6 +574,8 @@ int sc3d_moments(sc3d_ctx* c, const sc3d_bunch* b, double mean[6], double mom2[3
     const int64_t need = ((n + 4095) / 4096) * 21 + 64;
     if (need > c->M_cap) {
         dfree(c, c->M);
+        c->M = nullptr;
+        c->M_cap = 0;
         if (int rc = dmalloc(c, &c->M, need + 27)) return rc;
         c->M_cap = need;
     }

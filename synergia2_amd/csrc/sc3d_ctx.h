@@ -62,11 +62,21 @@ int fail(int code, const std::string& msg);
                                                   hipGetErrorString(e_));                  \
     } while (0)
 
+// Test hook: with the environment variable SC3D_TEST_MAX_ALLOC_BYTES set to
+// B, every workspace allocation of more than B bytes fails as a failed
+// hipMalloc would (before any device call), so tests can drive the error
+// paths; -1 when unset.
+int64_t test_alloc_limit();
+
 template <typename T>
 int dmalloc(sc3d_ctx* c, T** p, int64_t n)
 {
     void* q = nullptr;
     if (n <= 0) n = 1;
+    const int64_t lim = test_alloc_limit();
+    if (lim >= 0 && int64_t(n) * int64_t(sizeof(T)) > lim)
+        return ::sc3d::fail(SC3D_ERR_HIP, "hipMalloc failed: out of memory "
+                                          "(SC3D_TEST_MAX_ALLOC_BYTES)");
     SC3D_HIP_TRY(hipMalloc(&q, size_t(n) * sizeof(T)));
     c->owned.push_back(q);
     *p = static_cast<T*>(q);

@@ -158,18 +158,11 @@ struct sc3d_server {
 
 namespace {
 
+void drop_job_buffers(Job* j);
+
 void release_job(sc3d_server* s, Job* j)
 {
-    if (j->ctx) sc3d_destroy(j->ctx);
-    for (void* p : {(void*)j->parts, (void*)j->masks, (void*)j->ctl, (void*)j->d_progs,
-                    (void*)j->d_dt})
-        if (p) (void)hipFree(p);
-    j->ctx = nullptr;
-    j->parts = nullptr;
-    j->masks = nullptr;
-    j->ctl = nullptr;
-    j->d_progs = nullptr;
-    j->d_dt = nullptr;
+    drop_job_buffers(j);
     s->used -= j->footprint;
 }
 
@@ -190,7 +183,23 @@ void release_graph(sc3d_server* s)
     }
 }
 
-int admit(sc3d_server* s, Job* j)
+// Frees what an admission allocated for j (all of it, or the part that a
+// failed admission got to) and forgets the pointers.
+void drop_job_buffers(Job* j)
+{
+    if (j->ctx) sc3d_destroy(j->ctx);
+    for (void* p : {(void*)j->parts, (void*)j->masks, (void*)j->ctl, (void*)j->d_progs,
+                    (void*)j->d_dt})
+        if (p) (void)hipFree(p);
+    j->ctx = nullptr;
+    j->parts = nullptr;
+    j->masks = nullptr;
+    j->ctl = nullptr;
+    j->d_progs = nullptr;
+    j->d_dt = nullptr;
+}
+
+int admit_alloc(sc3d_server* s, Job* j)
 {
     const sc3d_job_spec& sp = j->spec;
     const int64_t n = sp.n_particles;
@@ -230,6 +239,20 @@ int admit(sc3d_server* s, Job* j)
     double mean[6], mom2[36];
     if (int rc = sc3d_moments(j->ctx, &b, mean, mom2, s->st)) return rc;
     emittances(mom2, j->emit0);
+    return SC3D_OK;
+}
+
+// Allocates and fills one job's device state.  On failure nothing of it
+// stays allocated and the job is marked failed (never retried).
+int admit(sc3d_server* s, Job* j)
+{
+    if (int rc = admit_alloc(s, j)) {
+        (void)hipStreamSynchronize(s->st);  // the populate may still be queued
+        drop_job_buffers(j);
+        j->state = SC3D_JOB_FAILED;
+        j->t_done = s->now();
+        return rc;
+    }
     j->ctx->last_stream = s->st;
     j->state = SC3D_JOB_RESIDENT;
     j->t_admit = s->now();
@@ -460,7 +483,13 @@ int sc3d_server_iterate(sc3d_server* s, int32_t* n_resident, int32_t* n_waiting)
                     break;
                 if (int rc = rebuild(s)) return rc;  // frees the finished jobs
             }
-            if (int rc = admit(s, j)) return rc;
+            if (int rc = admit(s, j)) {
+                // the jobs admitted before it stay resident; the failed one
+                // (marked SC3D_JOB_FAILED, nothing of it allocated) leaves
+                // the queue with them
+                s->waiting.erase(s->waiting.begin(), s->waiting.begin() + taken + 1);
+                return rc;
+            }
             s->resident.push_back(j);
             s->dirty = true;
             ++taken;

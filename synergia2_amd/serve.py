@@ -54,7 +54,7 @@ class ServerStats(C.Structure):
                 ("iter_ms_max", C.c_double)]
 
 
-JOB_WAITING, JOB_RESIDENT, JOB_DONE = 0, 1, 2
+JOB_WAITING, JOB_RESIDENT, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 _bound = False
 

@@ -9,6 +9,7 @@ bunch by cell).
 import numpy as np
 import pytest
 
+import oracle as O
 import synergia2_amd as sc
 from helpers import BETA, C1_SIGMA, GAMMA, by_id, gaussian_parts, oracle_run, rel_l2
 
@@ -382,23 +383,27 @@ def test_errors_and_empty(gpu):
         sc.SpaceCharge3dOpenHockney(sc.default_options(30, 32, 64))
 
 
-def test_populate_deterministic(gpu):
-    n = 200000
-    sig = np.array([1e-3, 2e-4, 3e-3, 4e-4, 1e-2, 1e-4])
-    b1 = sc.Bunch(n, 1e10, gamma=GAMMA)
+def test_populate_bit_exact_vs_oracle(gpu):
+    """k_populate (Philox-4x32-10, Box-Muller from basic operations, Cholesky
+    factor) equals the oracle's restatement bit for bit, with a correlated
+    factor, a mean, a 64-bit seed and ids above 2^32."""
+    n = 300000
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((6, 6)) * np.array([1e-3, 2e-4, 3e-3, 4e-4, 1e-2, 1e-4])[:, None]
+    L = np.linalg.cholesky(A @ A.T)
+    chol = np.array([L[r, c] for r in range(6) for c in range(r + 1)])
+    mean = np.array([1e-4, -2e-5, 3e-4, 0.0, -1e-3, 2e-6])
+    seed, id0 = (0xfedcba98 << 32) | 0x76543210, (1 << 32) - 1000
+    b = sc.Bunch(n, 1e10, gamma=GAMMA)
+    b.populate_gaussian(mean, chol, seed=seed, id0=id0)
+    got, m = b.host()
+    want, _ = O.populate_gaussian(n, mean, chol, seed, id0=id0)
+    assert np.all(m == 1)
+    assert np.array_equal(got, want)
+    # deterministic: a second fill of another bunch is the same
     b2 = sc.Bunch(n, 1e10, gamma=GAMMA)
-    b1.populate_gaussian(np.zeros(6), sc.diag_chol(sig), seed=42)
-    b2.populate_gaussian(np.zeros(6), sc.diag_chol(sig), seed=42)
-    p1, m1 = b1.host()
-    p2, _ = b2.host()
-    assert np.array_equal(p1, p2)
-    assert np.all(m1 == 1)
-    assert np.array_equal(p1[6], np.arange(n))
-    for j in range(6):
-        assert abs(p1[j].mean()) < 5 * sig[j] / np.sqrt(n)
-        assert abs(p1[j].std() / sig[j] - 1) < 0.01
-    cor = np.corrcoef(p1[:6])
-    assert np.abs(cor - np.eye(6)).max() < 0.01
+    b2.populate_gaussian(mean, chol, seed=seed, id0=id0)
+    assert np.array_equal(b2.host()[0], got)
 
 
 def test_fp32_grid_field_accuracy(gpu):
@@ -433,7 +438,6 @@ def test_fp32_grid_field_accuracy(gpu):
 def test_maps_parity(gpu):
     """Fused drift/quadrupole program vs the oracle's libFF restatement:
     bit-exact (same IEEE operations, sqrt and division correctly rounded)."""
-    import oracle as O
     from synergia2_amd import stepper as S
     n = 50000
     b = sc.Bunch(n, 1e11, gamma=GAMMA)
@@ -494,7 +498,6 @@ def torch_sync():
 def test_propagation_parity(gpu):
     """Six split-operator steps through the FODO channel (fused maps, sort
     every second kick) vs the oracle running the same sequence."""
-    import oracle as O
     from synergia2_amd import stepper as S
     n = 30000
     shape = (32, 32, 64)
@@ -532,7 +535,6 @@ def test_propagation_parity(gpu):
 def test_moments_bit_exact(gpu):
     """Device moments (mean, 6x6 second moments) equal the oracle's fixed-tree
     restatement bit for bit."""
-    import oracle as O
     from synergia2_amd import stepper as S
     n = 70001
     b = sc.Bunch(n, 1e10, gamma=GAMMA)
@@ -623,3 +625,37 @@ def test_convolution_vs_rocfft(gpu):
     err = rel_l2(phi, ref)
     print("phi vs rocFFT rel-L2", err)
     assert err <= 1e-12
+
+
+def test_apply_after_failed_growth(gpu, monkeypatch):
+    """A failed growth of the particle buffers (forced on the host with the
+    library's SC3D_TEST_MAX_ALLOC_BYTES hook, before any device work) leaves
+    no stale buffer behind: a later, smaller apply on the same context
+    reallocates and matches the oracle."""
+    shape = (32, 32, 64)
+    dt = 0.1 / (BETA * sc.C_LIGHT)
+    op = sc.SpaceCharge3dOpenHockney(make_opts(shape))
+    first = gaussian_parts(60000, C1_SIGMA, seed=71)
+    b1 = sc.Bunch(60000, 5e10, gamma=GAMMA)
+    b1.set_host(first, np.ones(60000, np.uint8))
+    op.apply_bunch(b1, dt)
+    big = sc.Bunch(400000, 5e10, gamma=GAMMA)
+    big.set_host(gaussian_parts(400000, C1_SIGMA, seed=72), np.ones(400000, np.uint8))
+    monkeypatch.setenv("SC3D_TEST_MAX_ALLOC_BYTES", str(8 << 20))
+    with pytest.raises(sc.Sc3dError):
+        op.apply_bunch(big, dt)
+    monkeypatch.delenv("SC3D_TEST_MAX_ALLOC_BYTES")
+    with pytest.raises(RuntimeError):
+        op.debug(sc.DBG_PERM, 60000)  # the old sort is gone with its buffers
+    n = 30000
+    parts = gaussian_parts(n, C1_SIGMA, seed=73)
+    masks = np.ones(n, np.uint8)
+    b = sc.Bunch(n, 5e10, gamma=GAMMA)
+    b.set_host(parts, masks)
+    op.apply_bunch(b, dt)
+    rc, out = oracle_run(oracle_kw(shape), parts, masks, 5e10, dt, want=("perm", "rho"))
+    assert rc == 0
+    assert np.array_equal(op.debug(sc.DBG_PERM, n), out["perm"])
+    assert rel_l2(op.debug(sc.DBG_RHO), out["rho"]) <= 1e-13
+    pg, mg = b.host()
+    check_kicked(pg, mg, out["parts"], parts)

@@ -154,3 +154,43 @@ def test_served_admission_respects_budget(gpu):
     with pytest.raises(sc.Sc3dError):
         small.submit(jobs[0])
     small.close()
+
+
+def test_failed_admission_leaves_server_consistent(gpu, monkeypatch):
+    """A job whose allocation fails at admission (forced on the host with the
+    library's SC3D_TEST_MAX_ALLOC_BYTES hook: its 128x128x256 spectrum
+    workspace exceeds the limit) is marked failed and dropped; the job
+    admitted before it in the same iteration stays resident once (stepped
+    once per iteration), the one behind it is admitted later, both end where
+    they end alone, and no memory stays accounted."""
+    def small(seed):
+        return SV.Job(20000, (32, 32, 64), 13, 1e-3, 1.2e-3, 0.010, seed=seed)
+    alone = SV.Server(SV.options(keep_particles=True))
+    ia = run_all(alone, [small(61), small(62)])
+    want = [alone.particles(i)[0] for i in ia]
+    alone.close()
+
+    big = SV.Job(30000, (128, 128, 256), 5, 2e-3, 2e-3, 0.02, seed=63)
+    srv = SV.Server(SV.options(keep_particles=True))
+    ids = [srv.submit(small(61)), srv.submit(big), srv.submit(small(62))]
+    monkeypatch.setenv("SC3D_TEST_MAX_ALLOC_BYTES", str(32 << 20))
+    with pytest.raises(sc.Sc3dError):
+        srv.iterate()
+    monkeypatch.delenv("SC3D_TEST_MAX_ALLOC_BYTES")
+    assert srv.result(ids[0]).state == SV.JOB_RESIDENT
+    assert srv.result(ids[1]).state == SV.JOB_FAILED
+    assert srv.result(ids[2]).state == SV.JOB_WAITING
+    st = srv.stats()
+    assert st["jobs_resident"] == 1 and st["jobs_waiting"] == 1
+    for _ in range(100):
+        res, wait = srv.iterate()
+        if res == 0 and wait == 0:
+            break
+    for k, jid in ((0, ids[0]), (1, ids[2])):
+        r = srv.result(jid)
+        assert r.state == SV.JOB_DONE and r.steps_done == 13
+        close(srv.particles(jid)[0], want[k], 1e-6)
+    assert srv.result(ids[1]).state == SV.JOB_FAILED
+    st = srv.stats()
+    srv.close()
+    assert st["jobs_done"] == 2 and st["bytes_resident"] == 0
