@@ -1,6 +1,6 @@
 """Throughput of the 3D open-Hockney space-charge step on one MI355X.
 
-Workload (the plain FP64 GPU configuration): a proton bunch at 0.8 GeV
+Main line (the plain FP64 GPU configuration): a proton bunch at 0.8 GeV
 kinetic energy, 4e6 macroparticles, population 1e11, 6D Gaussian matched to
 an O-F-O-D cell (0.5 m drifts, 0.2 m quads k1 = +-0.6 m^-2, rms emittance
 1 mm mrad), sigma_z = 10 mm, sigma_dp/p = 1e-4, seed 1; 4-cell channel with
@@ -19,8 +19,19 @@ loop, maps included, is reported beside it as the tracking rate.
 
 Also reported: the roofline of the dominant kernel (algorithmic bytes per
 launch over its HIP-event-timed duration, MI355X HBM 8 TB/s) and the CPU
-restatement of the reference operator (oracle/, OpenMP on the host cores)
-timed on a bounded sample of the same workload.
+restatement of the reference operator (oracle/, OpenMP on every host core
+granted to the process) timed on a bounded sample of the same workload.
+
+Sub-records of the same line (default run, no --config):
+  precision_form  1.6e7 particles, 128x128x256, FP32 deposit / FFT / field,
+                  FP64 particles: throughput, roofline, first-kick field
+                  rel-L2 against the oracle and 100-step emittance drift
+                  against the FP64 form;
+  served_form     the 256-job Poisson stream: saturated throughput, its ratio
+                  to the plain FP64 form on the same jobs, step and job
+                  p50 / p99 latency of the real-time stream;
+  drift_case      1e5 particles through a 1 m drift in 10 kicks on the GPU
+                  and on the CPU restatement beside it.
 """
 import argparse
 import json
@@ -92,7 +103,9 @@ def main():
     ap.add_argument("--steps", type=int, default=48,
                     help="timed steps (a multiple of the default sort period, 12)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="fodo_fp64_4M_64x64x128", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default=None, choices=sorted(CONFIGS),
+                    help="time only this workload (default: the FP64 workload as the main line, "
+                         "with the precision, served and drift-case sub-records)")
     ap.add_argument("--sort-period", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -109,6 +122,12 @@ def main():
                          "FODO ring (see long_tracking)")
     ap.add_argument("--turns", type=int, default=200, help="long: turns")
     ap.add_argument("--long-particles", type=int, default=10_000_000, help="long: particles")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="default run: skip the precision-form sub-record")
+    ap.add_argument("--no-served", action="store_true",
+                    help="default run: skip the served-form sub-record")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="default run: skip the drift-case (1e5 particles, CPU beside GPU) record")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a 1 GiB device-to-device copy (counter calibration and "
                          "achievable HBM bandwidth)")
@@ -135,44 +154,94 @@ def main():
 
     copy_gbps = copy_probe(torch) if args.copy_probe else None
 
-    cfg = CONFIGS[args.config]
+    main_cfg = args.config or "fodo_fp64_4M_64x64x128"
+    rec, bunch, opts, prop = timed_sc_run(args, sc, S, main_cfg, args.steps, args.warmup)
+    rec["roofline"]["copy_probe_GBps"] = copy_gbps
+    cfg = CONFIGS[main_cfg]
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(bunch, prop, opts, args.cpu_seconds, sc, S)
+    del bunch, prop
+    torch.cuda.empty_cache()
+
+    acc = None
+    if args.accuracy:
+        acc = accuracy_report(sc, S, main_cfg, 100)
+
+    out = {
+        "metric": "particle-kicks/s (3D Hockney SC step)",
+        "value": rec["value"],
+        "unit": "particle-kicks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": rec["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if cfg["precision"] == 0 else "f32-grid/f64-particles",
+        "data": "synthetic (seeded 6D Gaussian matched to the FODO cell)",
+        "config": rec["config"],
+        "roofline": rec["roofline"],
+        "cpu_baseline": cpu,
+    }
+    if acc is not None:
+        out["accuracy"] = acc
+    # the other forms of the headline metric, measured in the same run when
+    # the default workload is the main line: the precision form on its own
+    # workload (with its accuracy against the oracle), the served form on the
+    # job stream, and the small drift case on the CPU restatement beside the GPU
+    if args.config is None:
+        if not args.no_precision:
+            out["precision_form"] = precision_record(args, sc, S)
+            torch.cuda.empty_cache()
+        if not args.no_served:
+            out["served_form"] = served_record(args, sc, S)
+            torch.cuda.empty_cache()
+        if not args.no_c1:
+            out["drift_case"] = drift_case_record(sc, S)
+            torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+
+
+def timed_sc_run(args, sc, S, config, steps, warmup):
+    """Warm up, then time `steps` split-operator steps of a named workload:
+    the space-charge applies by HIP events per phase on the apply stream, the
+    whole loop (maps included) by the wall clock.  Returns the record, and
+    the bunch, options and propagator for the CPU leg."""
+    import torch
+    cfg = CONFIGS[config]
     n, grid, precision = cfg["n"], cfg["grid"], cfg["precision"]
-    bunch, opts, op, prop = make_workload(sc, S, args.config, sort_period=args.sort_period)
-
-    print(f"# {args.config}: n={n} grid={grid} warmup={args.warmup} steps={args.steps} "
+    bunch, opts, op, prop = make_workload(sc, S, config, sort_period=args.sort_period)
+    print(f"# {config}: n={n} grid={grid} warmup={warmup} steps={steps} "
           f"sort_period={opts.sort_period}", file=sys.stderr, flush=True)
-    prop.run(args.warmup, sync=True)
+    prop.run(warmup, sync=True)
     torch.cuda.synchronize()
-
     op.set_timing(True)
     op.timing(reset=True)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev0.record()
-    prop.run(args.steps, fuse=False)
-    ev1.record()
+    prop.run(steps, fuse=False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     op.sync()
     wall = t1 - t0
-    ev_ms = ev0.elapsed_time(ev1)
     phases, napplies = op.timing(reset=True)
-    assert napplies == args.steps
+    assert napplies == steps
     sc_s = sum(phases.values()) / 1e3
-    ms_per_step = 1e3 * sc_s / args.steps
-    value = n * args.steps / sc_s
+    ms_per_step = 1e3 * sc_s / steps
+    value = n * steps / sc_s
     print(f"# SC steps {sc_s:.4f}s -> {ms_per_step:.4f} ms/step, {value:.4e} particle-kicks/s; "
-          f"tracking wall {wall:.4f}s (events {ev_ms/1e3:.4f}s) incl. maps -> "
-          f"{n * args.steps / wall:.4e} /s", file=sys.stderr, flush=True)
+          f"tracking wall {wall:.4f}s incl. maps -> {n * steps / wall:.4e} /s",
+          file=sys.stderr, flush=True)
     for k, v in phases.items():
-        print(f"#   {k:16s} {v / args.steps * 1e3:9.2f} us/step", file=sys.stderr)
-
+        print(f"#   {k:16s} {v / steps * 1e3:9.2f} us/step", file=sys.stderr)
     w = 8 if precision == 0 else 4
-    traffic = measured_traffic(args.config)
+    traffic = measured_traffic(config)
     lines = []
     for phase, kname in (("sc3d_kick", "k_kick"), ("sc3d_local_rho", "k_deposit")):
-        per_launch_s = phases[phase] / args.steps / 1e3
+        per_launch_s = phases[phase] / steps / 1e3
         B = algorithmic_bytes(phase, n, grid, w)
         achieved = B / per_launch_s / 1e9
         lines.append({
@@ -188,47 +257,39 @@ def main():
         })
     # the dominant kernel: the longer of the two particle passes
     roofline = dict(max(lines, key=lambda r: r["launch_us"]))
-    roofline["copy_probe_GBps"] = copy_gbps
     roofline["particle_kernels"] = lines
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(bunch, prop, opts, args.cpu_seconds, sc, S)
-
-    acc = None
-    if args.accuracy:
-        del op, prop, bunch
-        torch.cuda.empty_cache()
-        acc = accuracy_report(sc, S, args.config, args.steps)
-
-    out = {
-        "metric": "particle-kicks/s (3D Hockney SC step)",
+    rec = {
         "value": value,
-        "unit": "particle-kicks/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if precision == 0 else "f32-grid/f64-particles",
-        "data": "synthetic (seeded 6D Gaussian matched to the FODO cell)",
         "config": {
-            "workload": args.config.replace("_", " ") + ", FODO 4 cells, 10 kicks/cell",
+            "workload": config.replace("_", " ") + ", FODO 4 cells, 10 kicks/cell",
             "particles": n, "grid": list(grid), "doubled_grid": [2 * x for x in grid],
             "green_fn": "linear", "domain": "adaptive", "sort_period": opts.sort_period,
-            "phase_us_per_step": {k: round(v / args.steps * 1e3, 2) for k, v in phases.items()},
-            "tracking_ms_per_step_incl_maps": round(1e3 * wall / args.steps, 4),
-            "tracking_particle_steps_per_s": n * args.steps / wall,
+            "phase_us_per_step": {k: round(v / steps * 1e3, 2) for k, v in phases.items()},
+            "tracking_ms_per_step_incl_maps": round(1e3 * wall / steps, 4),
+            "tracking_particle_steps_per_s": n * steps / wall,
         },
         "roofline": roofline,
-        "cpu_baseline": cpu,
     }
-    if acc is not None:
-        out["accuracy"] = acc
-    print(json.dumps(out), flush=True)
+    op.set_timing(False)
+    del op
+    return rec, bunch, opts, prop
 
+
+def precision_record(args, sc, S):
+    """The precision form (FP32 deposit / FFT / field, FP64 particles and
+    maps) on its own workload: 1.6e7 particles, 128x128x256, seed 2 --
+    throughput and roofline as the main line, then its accuracy: field
+    rel-L2 of the first kick against the oracle and 100-step rms-emittance
+    drift against the FP64 form."""
+    import torch
+    config = "fodo_fp32grid_16M_128x128x256"
+    rec, bunch, opts, prop = timed_sc_run(args, sc, S, config, args.steps, args.warmup)
+    del bunch, prop
+    torch.cuda.empty_cache()
+    rec["accuracy"] = accuracy_report(sc, S, config, 100)
+    rec["dtype"] = "f32-grid/f64-particles"
+    return rec
 
 def long_tracking(args, sc, S):
     """Long tracking: 1e7 macroparticles, 90 % 6D Gaussian matched to the
@@ -307,7 +368,7 @@ def long_tracking(args, sc, S):
     print(json.dumps(out), flush=True)
 
 
-def served_bench(args, sc, S):
+def served_record(args, sc, S):
     """Served form on the job stream: Poisson arrivals (mean 20 jobs/s) of
     `--jobs` FODO-channel jobs (seed 4; particles log-uniform in [1e4, 4e6];
     grid from {32x32x64, 64x64x128, 128x128x256}; steps uniform in [20, 200];
@@ -375,17 +436,14 @@ def served_bench(args, sc, S):
     wall_64 = time.perf_counter() - t0
     print(f"# fp64 plain: wall {wall_64:.2f} s -> {total / wall_64:.4e} particle-steps/s",
           file=sys.stderr, flush=True)
-
-    out = {
+    return {
         "metric": "particle-kicks/s (served FP32-grid form, Poisson job mix)",
         "value": total / wall_x,
         "unit": "particle-kicks/s",
-        "n_gpus": 1,
-        "higher_is_better": True,
         "dtype": "f32-grid/f64-particles",
-        "data": "synthetic (seeded job stream)",
-        "config": {"workload": f"{len(stream)} FODO jobs, Poisson 20/s, seed 4",
-                   "particle_steps": total},
+        "workload": f"{len(stream)} FODO jobs, Poisson 20/s, seed 4",
+        "particle_steps": total,
+        "vs_fp64_plain": (total / wall_x) / (total / wall_64),
         "saturated": {"wall_s": wall_x, "iterations": st_x["iterations"],
                       "graph_builds": st_x["graph_builds"],
                       "step_ms_p50": st_x["iter_ms_p50"], "step_ms_p99": st_x["iter_ms_p99"]},
@@ -395,9 +453,103 @@ def served_bench(args, sc, S):
                    "job_s_p50": float(np.percentile(lat_s, 50)),
                    "job_s_p99": float(np.percentile(lat_s, 99))},
         "fp64_plain": {"wall_s": wall_64, "throughput": total / wall_64},
-        "vs_fp64_plain": (total / wall_x) / (total / wall_64),
     }
+
+
+def served_bench(args, sc, S):
+    """The served form alone, as one JSON line of its own."""
+    rec = served_record(args, sc, S)
+    out = {"metric": rec["metric"], "value": rec["value"], "unit": rec["unit"], "n_gpus": 1,
+           "higher_is_better": True, "dtype": rec["dtype"],
+           "data": "synthetic (seeded job stream)",
+           "config": {"workload": rec["workload"], "particle_steps": rec["particle_steps"]}}
+    out.update({k: rec[k] for k in ("saturated", "stream", "fp64_plain", "vs_fp64_plain")})
     print(json.dumps(out), flush=True)
+
+
+def drift_case_record(sc, S):
+    """The small drift case: 1e5 particles (sigma 1 mm, 1e-4, 1 mm, 1e-4,
+    10 mm / beta, 1e-4; population 5e10, seed 0) through a 1 m drift in 10
+    split-operator kicks on 32x32x64, on the GPU (FP64) and on the CPU
+    restatement of the reference operator beside it (all granted cores),
+    with the largest coordinate difference after the 10 kicks."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n = 100000
+    gamma = (sc.PROTON_MASS + KINETIC_ENERGY) / sc.PROTON_MASS
+    beta, _ = sc.kinematics(sc.PROTON_MASS, gamma)
+    sig = (1e-3, 1e-4, 1e-3, 1e-4, 0.010 / beta, 1e-4)
+    b = sc.Bunch(n, 5e10, gamma=gamma)
+    b.populate_gaussian(np.zeros(6), sc.diag_chol(sig), seed=0)
+    p0, m0 = b.host()
+    op = sc.SpaceCharge3dOpenHockney(sc.default_options(32, 32, 64))
+    prop = S.Propagator([(sc.MAP_DRIFT, 1.0, 0.0)], 10, b, op)
+    prop.run(10, sync=True)  # warm-up pass on a copy of the state below
+    b.set_host(p0, m0)
+    prop.i = 0
+    op.close()
+    op = sc.SpaceCharge3dOpenHockney(sc.default_options(32, 32, 64))
+    prop = S.Propagator([(sc.MAP_DRIFT, 1.0, 0.0)], 10, b, op)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop.run(10, sync=True)
+    gpu_s = time.perf_counter() - t0
+    pg, _ = b.host()
+    po = np.ascontiguousarray(p0.copy())
+    mo = np.ascontiguousarray(m0.copy())
+    ob = O.make_bunch(po, mo, n, 5e10, 1, b.mass, b.gamma, b.beta, b.p)
+    cores = host_cores()
+    cpu_s = 0.0
+    for k in range(10):
+        first, second, length = prop.steps[k]
+        t0 = time.perf_counter()
+        O.apply_maps(ob, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, b.p, b.mass))
+                          for kd, L, k1 in first])
+        rc, _ = O.apply(O.make_options((32, 32, 64), reorder=(k == 0)), ob,
+                        length / (b.beta * sc.C_LIGHT), nthreads=cores["threads"])
+        assert rc == 0, O.last_error()
+        O.apply_maps(ob, [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, b.p, b.mass))
+                          for kd, L, k1 in second])
+        cpu_s += time.perf_counter() - t0
+    g = pg[:, np.argsort(pg[6], kind="stable")]
+    r = po[:, np.argsort(po[6], kind="stable")]
+    diff = max(float(np.abs(g[c] - r[c]).max() / np.abs(r[c]).max()) for c in range(6))
+    op.close()
+    print(f"# drift case: GPU {gpu_s * 1e3:.2f} ms, CPU {cpu_s * 1e3:.1f} ms for 10 steps "
+          f"(maps included), max coordinate difference {diff:.2e}", file=sys.stderr, flush=True)
+    return {"workload": "1e5 particles, 1 m drift in 10 split-operator kicks, 32x32x64, FP64, "
+                        "seed 0",
+            "gpu_ms_per_step": 1e3 * gpu_s / 10, "cpu_ms_per_step": 1e3 * cpu_s / 10,
+            "gpu_particle_kicks_per_s": n * 10 / gpu_s, "cpu_particle_kicks_per_s": n * 10 / cpu_s,
+            "cpu": cores, "max_rel_coordinate_diff_gpu_vs_cpu": diff}
+
+
+def host_cores():
+    """The host threads granted to this process: its CPU affinity set, bounded
+    by the cgroup CPU quota when one is set; with the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = min(aff, quota) if quota else aff
+    return {"threads": threads, "affinity": aff, "cgroup_quota_cpus": quota,
+            "nproc": os.cpu_count(), "model": model}
 
 
 def rel_l2(a, b):
@@ -431,7 +583,7 @@ def accuracy_report(sc, S, config, steps):
                               for kd, L, k1 in first])
             t0 = time.perf_counter()
             rc, out = O.apply(O.make_options(tuple(opts.shape), n_sigma=opts.n_sigma), ob,
-                              length / (bunch.beta * sc.C_LIGHT), nthreads=min(os.cpu_count(), 16),
+                              length / (bunch.beta * sc.C_LIGHT), nthreads=host_cores()["threads"],
                               want=("ex", "ey", "ez"))
             assert rc == 0, O.last_error()
             oracle_field = [out["ex"], out["ey"], out["ez"]]
@@ -507,7 +659,8 @@ def cpu_baseline(bunch, prop, opts, seconds, sc, S):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    cores = min(os.cpu_count() or 1, 16)
+    host = host_cores()
+    cores = host["threads"]
     parts, masks = bunch.host()
     parts = np.ascontiguousarray(parts)
     masks = np.ascontiguousarray(masks)
@@ -533,11 +686,13 @@ def cpu_baseline(bunch, prop, opts, seconds, sc, S):
         "value": bunch.n * steps / sc_time,
         "unit": "particle-kicks/s",
         "cores": cores,
+        "host": host,
         "kind": "port",
         "sample": f"{steps} space-charge steps of the same workload ({bunch.n} particles, "
                   f"grid {tuple(opts.shape)}, FODO channel; maps between kicks untimed), "
                   f"oracle/ CPU restatement of the reference operator with OpenMP ({cores} "
-                  f"threads) and its own FFT (FFTW absent); {sc_time:.2f} s of space-charge time",
+                  f"threads on {host['model']}; the reference binary cannot be built: no Kokkos, "
+                  f"FFTW absent, so its own FFT); {sc_time:.2f} s of space-charge time",
     }
 
 

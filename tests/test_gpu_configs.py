@@ -1,0 +1,373 @@
+"""The five workload configurations at their full sizes, on the GPU, against
+the oracle (CPU restatement of the reference operator, FP64):
+
+  C1  1e5 particles, 32x32x64, 10 split-operator kicks through a 1 m drift;
+  C2  4e6 particles matched to the FODO cell, 64x64x128, FP64, seed 1: the
+      first kick (domain, keys, permutation bit-exact; rho <= 1e-13; phi, E
+      <= 1e-12; kicked coordinates <= 1e-12 by id) and a kick in the middle
+      of a sort period (sort_period 12: the deposit works on windows of the
+      storage sorted at the first apply);
+  C3  1.6e7 particles, 128x128x256, seed 2: the FP64 form to the same bars
+      (the 256- and 512-long transforms), the precision form's field within
+      the 1e-5 bar (asserted at 2e-6; measured ~2e-7);
+  C4  1e7 particles, 90 % Gaussian + 10 % 3-sigma halo, 32-cell ring with one
+      kick per element, 128x128x256, seed 3: first-kick field of the
+      precision form and the tail fraction beyond 4 sigma;
+  C5  the 256-job Poisson stream (seed 4) in the served form: sampled jobs
+      end the same alone and in the mix, and one served job against the
+      oracle tracked step by step.
+
+Populate is bit-exact between the device and the oracle, so the oracle
+starts from its own copy of each bunch."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import synergia2_amd as sc
+from helpers import BETA, C1_SIGMA, GAMMA, by_id, rel_l2
+from synergia2_amd import serve as SV
+from synergia2_amd import stepper as S
+
+pytestmark = pytest.mark.gpu
+
+NT = max(1, min(16, len(os.sched_getaffinity(0))))
+DOMAIN_FIELDS = ("sums", "mean", "stddev", "size", "offset", "left", "h", "w0", "fn_norm",
+                 "factor")
+
+
+def check_domain(dg, do):
+    for f in DOMAIN_FIELDS:
+        a = np.array(dg[f], np.float64)
+        b = np.array(do[f], np.float64)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), (f, a, b)
+
+
+def slices_for(spec, b):
+    return [(kd, L, k1, 4, O.reference_cdt(kd, L, k1, 4, b.p, b.mass)) for kd, L, k1 in spec]
+
+
+def opts(shape, precision=sc.FP64, sort_period=12):
+    o = sc.default_options(*shape)
+    o.precision = precision
+    o.sort_period = sort_period
+    return o
+
+
+def compare_kicked(pg, po, before, cols=(1, 3, 5), tol=1e-12):
+    """Device vs oracle particles matched by id: positions identical, the
+    kicked columns within tol of the largest value, the kicks themselves
+    within 1e-10 rel-L2."""
+    g = by_id(pg)
+    r = po[:, np.argsort(po[6], kind="stable")]
+    p0 = before[:, np.argsort(before[6], kind="stable")]
+    assert np.array_equal(g[6], r[6])
+    for c in (0, 2, 4):
+        assert np.array_equal(g[c], r[c]), c
+    for c in cols:
+        err = np.abs(g[c] - r[c]).max() / np.abs(r[c]).max()
+        assert err <= tol, (c, err)
+        assert rel_l2(g[c] - p0[c], r[c] - p0[c]) <= 1e-10, c
+
+
+def oracle_bunch(parts, masks, b):
+    po = np.ascontiguousarray(parts.copy())
+    mo = np.ascontiguousarray(masks.copy())
+    return po, mo, O.make_bunch(po, mo, b.total_num, b.real_num, b.charge, b.mass, b.gamma,
+                                b.beta, b.p)
+
+
+def test_c1_drift_ten_kicks(gpu):
+    """C1 as specified: 1e5 particles (sigma 1 mm, 1e-4, 1 mm, 1e-4,
+    10 mm / beta, 1e-4; population 5e10, seed 0) through 1 m of drift in 10
+    split-operator steps (0.05 m drift, kick of 0.1 m, 0.05 m drift), grid
+    32x32x64, adaptive domain: every coordinate after the 10 kicks within
+    1e-12 of the oracle running the same sequence."""
+    n = 100000
+    b = sc.Bunch(n, 5e10, gamma=GAMMA)
+    b.populate_gaussian(np.zeros(6), sc.diag_chol(C1_SIGMA), seed=0)
+    p0, m0 = b.host()
+    po, mo, ob = oracle_bunch(p0, m0, b)
+    lattice = [(sc.MAP_DRIFT, 1.0, 0.0)]
+    op = sc.SpaceCharge3dOpenHockney(opts((32, 32, 64)))
+    prop = S.Propagator(lattice, 10, b, op)
+    print("case c1 n=1e5 10 kicks", flush=True)
+    prop.run(10, sync=True)
+    pg, _ = b.host()
+    for k in range(10):
+        first, second, length = prop.steps[k]
+        assert abs(length - 0.1) < 1e-15
+        O.apply_maps(ob, slices_for(first, b))
+        # the device reorders its storage on apply 0 (sort_period 12)
+        rc, _ = O.apply(O.make_options((32, 32, 64), reorder=(k == 0)), ob,
+                        length / (b.beta * sc.C_LIGHT), nthreads=NT)
+        assert rc == 0, O.last_error()
+        O.apply_maps(ob, slices_for(second, b))
+    g = by_id(pg)
+    r = po[:, np.argsort(po[6], kind="stable")]
+    assert np.array_equal(g[6], r[6])
+    for c in range(6):
+        err = np.abs(g[c] - r[c]).max() / np.abs(r[c]).max()
+        assert err <= 1e-12, (c, err)
+    # the space charge did something: x' grew beyond the drift-only value
+    assert np.abs(g[1] - by_id(p0)[1]).max() > 1e-7
+
+
+@pytest.mark.timeout(300)
+def test_c2_full_fp64_vs_oracle(gpu):
+    """C2 at full size: 4e6 particles matched to the O-F-O-D cell (rms
+    emittance 1 mm mrad, sigma_z 10 mm, sigma_dp/p 1e-4), population 1e11,
+    seed 1, grid 64x64x128, FP64, sort_period 12; the first split-operator
+    step's maps, then kick 0 (with the sort and reorder) and kick 4 (mid
+    period: windows of the storage sorted in LDS, no global sort)."""
+    n, shape = 4_000_000, (64, 64, 128)
+    chol = S.matched_chol(S.fodo_channel(1), 1e-6, 0.010 / BETA, 1e-4)
+    b = sc.Bunch(n, 1e11, gamma=GAMMA)
+    b.populate_gaussian(np.zeros(6), chol, seed=1)
+    p0, m0 = b.host()
+    po, mo, ob = oracle_bunch(p0, m0, b)
+    op = sc.SpaceCharge3dOpenHockney(opts(shape))
+    prop = S.Propagator(S.fodo_channel(4), 40, b, op)
+    for k in range(5):
+        first, second, length = prop.steps[k]
+        dt = length / (b.beta * sc.C_LIGHT)
+        if k == 0:
+            sc.apply_maps(b, prop.first[0])
+            O.apply_maps(ob, slices_for(first, b))
+        else:
+            sc.apply_maps(b, prop.between[k - 1])
+        check = k in (0, 4)
+        if check:
+            before, _ = b.host()
+            if k == 0:  # the maps agree bit for bit at full size
+                assert np.array_equal(before, po)
+            else:
+                # the oracle kicks the device's own state (the kicks before
+                # differ in the last bits, and the maps carry that into the
+                # positions); storage order as on the device, which reordered
+                # it on apply 0
+                po[:] = before
+        op.apply_bunch(b, dt)
+        if not check:
+            continue
+        print(f"case c2 kick={k}", flush=True)
+        want = ("keys", "perm", "rho", "phi", "ex", "ey", "ez")
+        rc, out = O.apply(O.make_options(shape, reorder=(k == 0)), ob, dt, nthreads=NT,
+                          want=want)
+        assert rc == 0, O.last_error()
+        check_domain(op.debug(sc.DBG_DOMAIN), out["domain"])
+        if k == 0:
+            assert np.array_equal(op.debug(sc.DBG_KEYS, n), out["keys"])
+            assert np.array_equal(op.debug(sc.DBG_PERM, n), out["perm"])
+        assert rel_l2(op.debug(sc.DBG_RHO), out["rho"]) <= 1e-13
+        assert rel_l2(op.debug(sc.DBG_PHI), out["phi"]) <= 1e-12
+        for tap, name in ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez")):
+            assert rel_l2(op.debug(tap), out[name]) <= 1e-12, name
+        pg, _ = b.host()
+        assert np.array_equal(pg[6], po[6])  # same storage order as the oracle
+        compare_kicked(pg, po, before)
+
+
+def c3_bunch():
+    n = 16_000_000
+    chol = S.matched_chol_sigma(S.fodo_channel(1), 2e-3, 2e-3, 0.020 / BETA, 1e-4)
+    b = sc.Bunch(n, 2e11, gamma=GAMMA)
+    b.populate_gaussian(np.zeros(6), chol, seed=2)
+    return b
+
+
+@pytest.mark.timeout(300)
+def test_c3_full_both_forms_vs_oracle(gpu):
+    """C3 at full size: 1.6e7 particles, sigma_x = sigma_y = 2 mm (matched),
+    sigma_z 20 mm, population 2e11, seed 2, grid 128x128x256 (doubled
+    256x256x512), first kick after the first maps.  FP64 form: the FP64 bars;
+    precision form (FP32 deposit / FFT / field): field rel-L2 <= 2e-6 (bar
+    1e-5), density <= 5e-7, kicks <= 1e-5 rel-L2."""
+    shape = (128, 128, 256)
+    b64 = c3_bunch()
+    prop = S.Propagator(S.fodo_channel(4), 40, b64, None)
+    first, _, length = prop.steps[0]
+    dt = length / (b64.beta * sc.C_LIGHT)
+    sc.apply_maps(b64, prop.first[0])
+    before, m0 = b64.host()
+    po, mo, ob = oracle_bunch(before, m0, b64)
+    print("case c3 oracle", flush=True)
+    rc, out = O.apply(O.make_options(shape, reorder=True), ob, dt, nthreads=NT,
+                      want=("rho", "phi", "ex", "ey", "ez"))
+    assert rc == 0, O.last_error()
+    print("case c3 fp64", flush=True)
+    op = sc.SpaceCharge3dOpenHockney(opts(shape))
+    op.apply_bunch(b64, dt)
+    check_domain(op.debug(sc.DBG_DOMAIN), out["domain"])
+    assert rel_l2(op.debug(sc.DBG_RHO), out["rho"]) <= 1e-13
+    assert rel_l2(op.debug(sc.DBG_PHI), out["phi"]) <= 1e-12
+    for tap, name in ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez")):
+        assert rel_l2(op.debug(tap), out[name]) <= 1e-12, name
+    pg, _ = b64.host()
+    compare_kicked(pg, po, before)
+    del op, b64, pg
+
+    print("case c3 fp32 grid", flush=True)
+    b32 = c3_bunch()
+    sc.apply_maps(b32, prop.first[0])
+    op32 = sc.SpaceCharge3dOpenHockney(opts(shape, precision=sc.FP32_GRID))
+    op32.apply_bunch(b32, dt)
+    errs = [rel_l2(op32.debug(t), out[nm]) for t, nm in
+            ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez"))]
+    rho_err = rel_l2(op32.debug(sc.DBG_RHO), out["rho"])
+    print(f"  C3 precision form: field rel-L2 {errs}, rho {rho_err:.2e}", flush=True)
+    assert max(errs) <= 2e-6
+    assert rho_err <= 5e-7
+    g = by_id(b32.host()[0])
+    r = po[:, np.argsort(po[6], kind="stable")]
+    p0 = before[:, np.argsort(before[6], kind="stable")]
+    for c in (1, 3, 5):
+        assert rel_l2(g[c] - p0[c], r[c] - p0[c]) <= 1e-5, c
+
+
+@pytest.mark.timeout(300)
+def test_c4_halo_first_kick_and_tails(gpu):
+    """C4: 1e7 particles, 90 % Gaussian matched to the cell (2 mm, 2 mm,
+    20 mm) + 10 % uniform in the transverse 4D ellipsoid at 3 sigma, seed 3,
+    32-cell ring with one kick per element, grid 128x128x256, precision form:
+    the first kick's field within 2e-6 rel-L2 of the oracle (bar 1e-5), the
+    kicks within 1e-5, and the device's tail fraction beyond 4 sigma equal
+    to numpy's count on the same particles."""
+    n, shape = 10_000_000, (128, 128, 256)
+    chol = S.matched_chol_sigma(S.fodo_channel(1), 2e-3, 2e-3, 0.020 / BETA, 1e-4)
+    parts = S.halo_mix_parts(n, chol, seed=3)
+    masks = np.ones(n, np.uint8)
+    steps = S.split_operator_steps_elements(S.fodo_channel(32), 1)
+    assert len(steps) == 128
+    b = sc.Bunch(n, 2e11, gamma=GAMMA)
+    b.set_host(parts, masks)
+    first, _, length = steps[0]
+    dt = length / (b.beta * sc.C_LIGHT)
+    sc.apply_maps(b, sc.make_slices(first, b.p, b.mass))
+    before, _ = b.host()
+    po, mo, ob = oracle_bunch(before, masks, b)
+    print("case c4 oracle", flush=True)
+    rc, out = O.apply(O.make_options(shape, reorder=True), ob, dt, nthreads=NT,
+                      want=("ex", "ey", "ez"))
+    assert rc == 0, O.last_error()
+    op = sc.SpaceCharge3dOpenHockney(opts(shape, precision=sc.FP32_GRID))
+    op.apply_bunch(b, dt)
+    errs = [rel_l2(op.debug(t), out[nm]) for t, nm in
+            ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez"))]
+    print(f"  C4 first-kick field rel-L2 {errs}", flush=True)
+    assert max(errs) <= 2e-6
+    pg, _ = b.host()
+    g = by_id(pg)
+    r = po[:, np.argsort(po[6], kind="stable")]
+    p0 = before[:, np.argsort(before[6], kind="stable")]
+    for c in (1, 3, 5):
+        assert rel_l2(g[c] - p0[c], r[c] - p0[c]) <= 1e-5, c
+    mean, mom2 = op.moments(b)
+    width = np.sqrt(np.diag(mom2))
+    got = op.tail_fraction(b, mean, width, 4.0)
+    want = [np.count_nonzero(np.abs(pg[j] - mean[j]) > 4.0 * width[j]) / n for j in range(6)]
+    assert np.array_equal(got, np.array(want))
+    # the halo population is what reaches beyond 4 sigma transversely
+    assert 1e-4 < got[0] < 5e-2 and 1e-4 < got[2] < 5e-2
+
+
+def served_vs_oracle_job(stream):
+    """The stream's job that is cheapest to track on the oracle (grid
+    32x32x64, fewest particle-steps)."""
+    best = None
+    for i, (_, kw) in enumerate(stream):
+        if tuple(kw["shape"]) != (32, 32, 64):
+            continue
+        cost = kw["n"] * kw["steps"]
+        if best is None or cost < best[0]:
+            best = (cost, i)
+    return best[1]
+
+
+@pytest.mark.timeout(600)
+def test_c5_stream_alone_vs_mix_and_oracle(gpu):
+    """C5: the 256-job stream (Poisson 20 jobs/s, seed 4; particles
+    log-uniform in [1e4, 4e6], grids 32x32x64 / 64x64x128 / 128x128x256,
+    20-200 steps), all submitted at once to the served precision form.
+    Six sampled jobs (the smallest, the largest, the median of each grid)
+    end with the same second moments and emittances when served alone
+    (<= 1e-6 relative; the FP32 atomics order is the only difference); one
+    job, tracked step by step by the FP64 oracle from the same populated
+    bunch, ends with its emittances within 1e-4 and its coordinates within
+    1e-5 of the served result."""
+    stream = SV.job_stream(count=256, seed=4)
+    srv = SV.Server(SV.options())
+    ids = [srv.submit(SV.Job(**kw)) for _, kw in stream]
+    print("case c5 mix of 256", flush=True)
+    for it in range(100000):
+        res, wait = srv.iterate()
+        if res == 0 and wait == 0:
+            break
+    st = srv.stats()
+    mix = [srv.result(i) for i in ids]
+    srv.close()
+    assert st["jobs_done"] == 256
+    assert all(r.state == SV.JOB_DONE and r.steps_done == kw["steps"]
+               for r, (_, kw) in zip(mix, stream))
+    ns = [kw["n"] for _, kw in stream]
+    sample = {int(np.argmin(ns)), int(np.argmax(ns))}
+    for shape in ((32, 32, 64), (64, 64, 128), (128, 128, 256)):
+        idx = [i for i, (_, kw) in enumerate(stream) if tuple(kw["shape"]) == shape]
+        idx.sort(key=lambda i: ns[i])
+        sample.add(idx[len(idx) // 2])
+    for i in sorted(sample):
+        print(f"case c5 alone job {i}: n={ns[i]} {stream[i][1]['shape']}", flush=True)
+        one = SV.Server(SV.options())
+        jid = one.submit(SV.Job(**stream[i][1]))
+        while True:
+            res, wait = one.iterate()
+            if res == 0 and wait == 0:
+                break
+        ra = one.result(jid)
+        one.close()
+        rm = mix[i]
+        for k in range(3):
+            assert ra.emit0[k] == rm.emit0[k]
+            assert abs(rm.emit[k] / ra.emit[k] - 1) <= 1e-6, (i, k)
+        for j in range(6):
+            assert abs(rm.mom2[7 * j] / ra.mom2[7 * j] - 1) <= 1e-6, (i, j)
+
+    i = served_vs_oracle_job(stream)
+    kw = stream[i][1]
+    print(f"case c5 served vs oracle job {i}: n={kw['n']} steps={kw['steps']}", flush=True)
+    job = SV.Job(**kw)
+    one = SV.Server(SV.options(keep_particles=True))
+    jid = one.submit(job)
+    while True:
+        res, wait = one.iterate()
+        if res == 0 and wait == 0:
+            break
+    got, _ = one.particles(jid)
+    rs = one.result(jid)
+    one.close()
+    parts, masks = O.populate_gaussian(job.n, np.zeros(6), job.chol, job.seed)
+    ob = O.make_bunch(parts, masks, job.n, job.real_num, 1, job.mass, job.gamma, job.beta, job.p)
+    steps = S.split_operator_steps(job.lattice, job.nsteps)
+
+    class Ref:
+        p, mass = job.p, job.mass
+    e0 = sc.emittances(O.moments(parts, masks)[1])
+    for k in range(job.steps):
+        first, second, length = steps[k % len(steps)]
+        O.apply_maps(ob, slices_for(first, Ref))
+        rc, _ = O.apply(O.make_options(job.shape), ob, length / (job.beta * sc.C_LIGHT),
+                        nthreads=NT)
+        assert rc == 0, O.last_error()
+        O.apply_maps(ob, slices_for(second, Ref))
+    e1 = sc.emittances(O.moments(parts, masks)[1])
+    print(f"  served emit {list(rs.emit)} oracle {e1} (initial {e0})", flush=True)
+    for k in range(3):
+        assert rs.emit0[k] == pytest.approx(e0[k], rel=1e-12)
+        assert abs(rs.emit[k] / e1[k] - 1) <= 1e-4, k
+    g = by_id(got)
+    r = parts[:, np.argsort(parts[6], kind="stable")]
+    assert np.array_equal(g[6], r[6])
+    errs = [np.abs(g[c] - r[c]).max() / np.abs(r[c]).max() for c in range(6)]
+    print(f"  served vs oracle coordinates {errs}", flush=True)
+    assert max(errs) <= 1e-5
