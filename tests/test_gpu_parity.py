@@ -659,3 +659,28 @@ def test_apply_after_failed_growth(gpu, monkeypatch):
     assert rel_l2(op.debug(sc.DBG_RHO), out["rho"]) <= 1e-13
     pg, mg = b.host()
     check_kicked(pg, mg, out["parts"], parts)
+
+
+def test_fixture_bunch_1k(gpu):
+    """The HIP path against the committed fixture (tests/golden/bunch_1k.npz,
+    the oracle's outputs on a seeded 1e3-particle bunch with masked and
+    out-of-grid particles, 8x8x16): domain record, keys and permutation
+    exactly, rho <= 1e-13, phi and E <= 1e-12, kicked bunch <= 1e-12 by id."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                              "bunch_1k.npz"))
+    parts, masks = fx["parts_in"], fx["masks"]
+    n = masks.shape[0]
+    shape = tuple(int(x) for x in fx["shape"])
+    op, b = gpu_run(make_opts(shape), parts, masks, float(fx["real_num"]), float(fx["time_step"]))
+    dom = op.debug(sc.DBG_DOMAIN)
+    for f in DOMAIN_FIELDS:
+        assert np.array_equal(np.array(dom[f], np.float64), fx["domain_" + f]), f
+    assert np.array_equal(op.debug(sc.DBG_KEYS, n), fx["keys"])
+    assert np.array_equal(op.debug(sc.DBG_PERM, n), fx["perm"])
+    assert rel_l2(op.debug(sc.DBG_RHO), fx["rho"]) <= 1e-13
+    assert rel_l2(op.debug(sc.DBG_PHI), fx["phi"]) <= 1e-12
+    for tap, name in ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez")):
+        assert rel_l2(op.debug(tap), fx[name]) <= 1e-12, name
+    pg, mg = b.host()
+    check_kicked(pg, mg, fx["parts_out"], parts)
