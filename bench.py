@@ -117,6 +117,10 @@ def main():
                     help="measure the served form on the Poisson job stream instead (see "
                          "served_bench)")
     ap.add_argument("--jobs", type=int, default=256, help="served: jobs in the stream")
+    ap.add_argument("--form", choices=("served", "plain64", "plain32"), default=None,
+                    help="run only one form on the job stream, all jobs at once (served) or "
+                         "one after another through the operator API (plain64 / plain32), and "
+                         "print its wall time: the unit of the per-form kernel-time split")
     ap.add_argument("--long", action="store_true",
                     help="long tracking of the precision form against the FP64 form on the "
                          "FODO ring (see long_tracking)")
@@ -145,6 +149,9 @@ def main():
     import synergia2_amd as sc
     from synergia2_amd import stepper as S
 
+    if args.form:
+        form_run(args, sc, S)
+        return
     if args.served:
         served_bench(args, sc, S)
         return
@@ -454,6 +461,45 @@ def served_record(args, sc, S):
                    "job_s_p99": float(np.percentile(lat_s, 99))},
         "fp64_plain": {"wall_s": wall_64, "throughput": total / wall_64},
     }
+
+
+def form_run(args, sc, S):
+    """One form alone on the first `--jobs` jobs of the stream (seed 4): the
+    served form with every job submitted at t = 0, or the plain operator
+    form job after job (FP64 or FP32 grid).  Under rocprofv3 --kernel-trace
+    this gives each form's GPU time per kernel on the same job mix."""
+    import numpy as np
+    import torch
+    from synergia2_amd import serve as SV
+    stream = SV.job_stream(count=args.jobs, seed=4)
+    total = sum(kw["n"] * kw["steps"] for _, kw in stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if args.form == "served":
+        srv = SV.Server(SV.options())
+        for _, kw in stream:
+            srv.submit(SV.Job(**kw))
+        while True:
+            res, wait = srv.iterate()
+            if res == 0 and wait == 0:
+                break
+        srv.close()
+    else:
+        prec = sc.FP64 if args.form == "plain64" else sc.FP32_GRID
+        for _, kw in stream:
+            job = SV.Job(**kw)
+            b = sc.Bunch(job.n, job.real_num, gamma=job.gamma)
+            b.populate_gaussian(np.zeros(6), job.chol, seed=job.seed)
+            o = sc.default_options(*job.shape)
+            o.precision = prec
+            op = sc.SpaceCharge3dOpenHockney(o)
+            S.Propagator(job.lattice, job.nsteps, b, op).run(job.steps, sync=True)
+            op.close()
+            del b
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(json.dumps({"form": args.form, "jobs": len(stream), "particle_steps": total,
+                      "wall_s": wall, "particle_steps_per_s": total / wall}), flush=True)
 
 
 def served_bench(args, sc, S):

@@ -95,8 +95,10 @@ typedef struct sc3d_ctx sc3d_ctx;
 void sc3d_options_default(sc3d_options* opt, int32_t gx, int32_t gy, int32_t gz);
 
 /* Creates a context on the current HIP device: validates the options and
- * allocates the grid workspaces. Shapes must be even, each doubled length a
- * power of two in [8, 512]. */
+ * allocates the grid workspaces.  Any shape with every axis in [4, 512]
+ * (doubled lengths up to 1024) and fewer than 2^31 cells; power-of-two axes
+ * (x in [8, 256], doubled y and z in [8, 512]) run the tuned transform
+ * passes, other lengths generic mixed-radix ones. */
 int sc3d_create(const sc3d_options* opt, sc3d_ctx** out);
 
 /* Enqueues one space-charge kick of length time_step [s] on `stream`
@@ -229,6 +231,18 @@ int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n,
                            uint8_t* masks, const double mean[6],
                            const double chol[21], uint64_t seed, int64_t id0,
                            void* stream);
+
+/* Host only (no device call).  The points of the doubled-grid Green's
+ * function where the reference's flat-index decomposition in alg_g2_linear /
+ * alg_g2_pointlike (space_charge_3d_open_hockney.cc:170-176,272-278) leaves
+ * another value than the symmetric one at this shape, replayed in its serial
+ * write order; sc3d_apply adds their effect, so the field is the reference's
+ * at every shape.  Writes min(count, cap) records of 6 int32 -- target
+ * (z, y, x), then the (iz, iy, ix) whose value stands there, or -1 -1 -1 for
+ * a point left at zero -- and returns the count (<0: error).  No reference
+ * counterpart: it exposes that behaviour for tests and callers. */
+int sc3d_green_patches(int32_t green_fn, const int32_t shape[3], int32_t* out,
+                       int32_t cap);
 
 #ifdef __cplusplus
 }

@@ -124,7 +124,7 @@ PHASES = ("sc3d_domain", "sc3d_sort", "sc3d_local_rho", "sc3d_green_fn2", "sc3d_
 EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
            "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
-           "sc3d_moments", "sc3d_tail_fraction",
+           "sc3d_moments", "sc3d_tail_fraction", "sc3d_green_patches",
            # served form (include/sc3d_serve.h)
            "sc3d_server_options_default", "sc3d_server_create", "sc3d_server_submit",
            "sc3d_server_iterate", "sc3d_server_job", "sc3d_server_job_particles",
@@ -166,6 +166,9 @@ def load(path=None):
     L.sc3d_tail_fraction.argtypes = [C.c_void_p, C.POINTER(BunchDesc), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.c_double,
                                      C.POINTER(C.c_double), C.c_void_p]
+    L.sc3d_green_patches.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.c_int32]
+    L.sc3d_green_patches.restype = C.c_int
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
                  "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
                  "sc3d_get_timing", "sc3d_set_post_kick_maps", "sc3d_moments",
@@ -195,6 +198,20 @@ def set_fixed_domain(opts, offset, size):
     opts.offset[:] = list(offset)
     opts.size[:] = list(size)
     opts.domain_fixed = 1
+
+
+def green_patches(green_fn, shape):
+    """The points where the reference's g2 index decomposition leaves another
+    value than the symmetric one at this shape (host only): an (n, 6) int32
+    array of target (z, y, x) and the writer's (iz, iy, ix), -1 for none."""
+    L = load()
+    shp = (C.c_int32 * 3)(*shape)
+    n = L.sc3d_green_patches(int(green_fn), shp, None, 0)
+    if n < 0:
+        _check(n)
+    out = np.zeros((max(n, 1), 6), np.int32)
+    L.sc3d_green_patches(int(green_fn), shp, out.ctypes.data_as(C.POINTER(C.c_int32)), n)
+    return out[:n]
 
 
 def kinematics(mass, gamma):

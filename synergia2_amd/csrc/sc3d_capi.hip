@@ -21,8 +21,6 @@ using namespace sc3d;
 namespace {
 thread_local std::string g_err;
 
-bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
-
 #define HIP_TRY SC3D_HIP_TRY
 
 // Releases the particle-side buffers and forgets them: no pointer to freed
@@ -64,26 +62,38 @@ int ensure_particle_buffers(sc3d_ctx* c, int64_t n)
     return 0;
 }
 
-void make_twiddles(std::vector<double2>& t64, std::vector<float2>& t32)
+// e^{-2 pi i k / L}, k < L, appended to t64 / t32 (long double, exact on the
+// axes where L allows)
+void append_twiddles(int L, std::vector<double2>& t64, std::vector<float2>& t32)
 {
-    t64.resize(kTwiddleElems);
-    t32.resize(kTwiddleElems);
     const long double pi = 3.141592653589793238462643383279502884L;
-    for (int L = 1; L <= kMaxLen; L *= 2)
-        for (int k = 0; k < L; ++k) {
-            long double a = -2.0L * pi * k / L;
-            long double cr = std::cos(a), si = std::sin(a);
-            // exact values on the axes
-            if (4 * k == L) { cr = 0; si = -1; }
-            if (2 * k == L) { cr = -1; si = 0; }
-            if (4 * k == 3 * L) { cr = 0; si = 1; }
-            if (k == 0) { cr = 1; si = 0; }
-            t64[L - 1 + k] = make_double2(double(cr), double(si));
-            t32[L - 1 + k] = make_float2(float(cr), float(si));
-        }
+    for (int k = 0; k < L; ++k) {
+        long double a = -2.0L * pi * k / L;
+        long double cr = std::cos(a), si = std::sin(a);
+        if (4 * k == L) { cr = 0; si = -1; }
+        if (2 * k == L) { cr = -1; si = 0; }
+        if (4 * k == 3 * L) { cr = 0; si = 1; }
+        if (k == 0) { cr = 1; si = 0; }
+        t64.push_back(make_double2(double(cr), double(si)));
+        t32.push_back(make_float2(float(cr), float(si)));
+    }
 }
 
-Geometry geometry_of(const sc3d_options& o)
+// The power-of-two tables (twiddle_offset), then the tables of the
+// generic axes of g at g.tw_any.
+void make_twiddles(const Geometry& g, std::vector<double2>& t64, std::vector<float2>& t32)
+{
+    t64.clear();
+    t32.clear();
+    for (int L = 1; L <= kMaxLen; L *= 2) append_twiddles(L, t64, t32);
+    const int len[3] = {g.Dx, g.Dy, g.Dz};
+    for (int a = 0; a < 3; ++a)
+        if (!g.tuned[a]) append_twiddles(len[a], t64, t32);
+}
+
+// all_generic: every axis on the generic passes (the shapes whose g2
+// correction is a spectrum, which only those passes apply)
+Geometry geometry_of(const sc3d_options& o, bool all_generic = false)
 {
     Geometry g;
     g.gx = o.shape[0]; g.gy = o.shape[1]; g.gz = o.shape[2];
@@ -92,6 +102,21 @@ Geometry geometry_of(const sc3d_options& o)
     g.G = int64_t(g.gx) * g.gy * g.gz;
     g.S_elems = int64_t(g.gz) * g.Dy * g.Hx;
     g.Ghat_elems = int64_t(g.gz + 1) * (g.gy + 1) * g.Hx;
+    g.tuned[0] = !all_generic && tuned_row_length(g.gx);
+    g.tuned[1] = !all_generic && tuned_col_length(g.Dy);
+    g.tuned[2] = !all_generic && tuned_col_length(g.Dz);
+    const int len[3] = {g.Dx, g.Dy, g.Dz};
+    int64_t off = kTwiddleElems;
+    for (int a = 0; a < 3; ++a) {
+        g.plan[a] = AnyPlan{};
+        g.tw_any[a] = -1;
+        if (!g.tuned[a]) {
+            g.plan[a] = any_plan(len[a]);
+            g.tw_any[a] = off;
+            off += len[a];
+        }
+    }
+    g.tw_elems = off;
     return g;
 }
 
@@ -137,16 +162,21 @@ hipEvent_t mark(sc3d_ctx* c, hipStream_t st)
 
 int64_t ctx_footprint(const sc3d_options& o, int64_t n)
 {
-    const Geometry g = geometry_of(o);
+    std::vector<GreenPatch> patches;
+    const int np = green_patches(o.green_fn, o.shape, patches);
+    const bool spectral = np > kMaxGreenPatches;
+    const Geometry g = geometry_of(o, spectral);
     const bool fp32 = o.precision == SC3D_PRECISION_FP32_GRID;
     const int64_t rs = fp32 ? 4 : 8;
     int64_t b = sizeof(DomainRec) + sizeof(GridJob) + sizeof(StepCtl);
+    b += int64_t(np) * sizeof(GreenPatch);
+    if (spectral) b += int64_t(g.Dz) * g.Dy * (g.Dx + 2 * g.Hx) * rs;  // C + chat
     b += g.G * rs;                               // rho
     b += g.S_elems * 2 * rs * (fp32 ? 2 : 1);    // S (+ S2)
     b += fp32 ? 0 : g.G * rs;                    // phi
     b += g.G * 4 * rs;                           // E
     b += g.Ghat_elems * (fp32 ? 4 : 8);          // Green's spectrum
-    b += kTwiddleElems * (16 + 8);               // twiddles
+    b += g.tw_elems * (16 + 8);                  // twiddles
     const int64_t cap = n + n / 8 + 4096;        // ensure_particle_buffers
     const int64_t tiles = (cap + kSortTileItems - 1) / kSortTileItems;
     b += cap * 4 * 5 + 2048 * tiles * 4 + 2048 * 4 + cap * 8 * 7 + cap;
@@ -253,21 +283,25 @@ void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream
     const Geometry& g = c->g;
     if (!c->fp32) {
         launch_green<double>(g, c->opt.green_fn, table, nj, c->tw64, st);
+        for (int j = 0; j < nj; ++j) launch_green_chat(ctxs[j], st);
         ev[4] = mark(c, st);
         launch_solve<double>(g, table, nj, c->tw64, st);
         ev[5] = mark(c, st);
         for (int j = 0; j < nj; ++j) {
             sc3d_ctx* cj = ctxs[j];
+            launch_green_patches(cj, st);
             launch_efield<double>(static_cast<double*>(cj->phi), g.gx, g.gy, g.gz, cj->dom,
                                   static_cast<double4*>(cj->E), st);
         }
         ev[6] = mark(c, st);
     } else {
         launch_green<float>(g, c->opt.green_fn, table, nj, c->tw32, st);
+        for (int j = 0; j < nj; ++j) launch_green_chat(ctxs[j], st);
         ev[4] = mark(c, st);
         // the field comes straight out of the spectrum (no potential, no
         // separate difference pass): sc3d_force stays empty in this form
         launch_solve_field<float>(g, table, nj, c->tw32, st);
+        for (int j = 0; j < nj; ++j) launch_green_patches(ctxs[j], st);
         ev[5] = mark(c, st);
         ev[6] = mark(c, st);
     }
@@ -321,10 +355,13 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     if (!opt || !out) return fail(SC3D_ERR_ARG, "sc3d_create: null argument");
     *out = nullptr;
     const int gx = opt->shape[0], gy = opt->shape[1], gz = opt->shape[2];
-    if (!pow2_in(gx, 8, 256) || !pow2_in(gy, 4, 256) || !pow2_in(gz, 4, 256))
-        return fail(SC3D_ERR_ARG,
-                    "sc3d_create: grid shape must be powers of two with gx in [8,256] and "
-                    "gy, gz in [4,256]");
+    if (gx < 4 || gy < 4 || gz < 4 || 2 * gx > kAnyMaxLen || 2 * gy > kAnyMaxLen ||
+        2 * gz > kAnyMaxLen)
+        return fail(SC3D_ERR_ARG, "sc3d_create: grid shape " + std::to_string(gx) + "x" +
+                                      std::to_string(gy) + "x" + std::to_string(gz) +
+                                      ": each axis must be in [4, 512] (doubled length <= 1024)");
+    if (int64_t(gx) * gy * gz >= (int64_t(1) << 31))
+        return fail(SC3D_ERR_ARG, "sc3d_create: more than 2^31 grid cells");
     if (opt->green_fn != SC3D_GREEN_LINEAR && opt->green_fn != SC3D_GREEN_POINTLIKE)
         return fail(SC3D_ERR_ARG, "sc3d_create: unknown green_fn");
     if (opt->precision != SC3D_PRECISION_FP64 && opt->precision != SC3D_PRECISION_FP32_GRID)
@@ -333,7 +370,13 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     sc3d_ctx* c = new sc3d_ctx();
     c->opt = *opt;
     c->fp32 = opt->precision == SC3D_PRECISION_FP32_GRID;
-    c->g = geometry_of(*opt);
+    // the shapes where the reference's g2 index decomposition misplaces
+    // points get its values there (sc3d_quirk.hip): few points in direct
+    // space, many as a correction spectrum on the generic passes
+    std::vector<GreenPatch> patches;
+    c->npatch = green_patches(opt->green_fn, opt->shape, patches);
+    c->spectral_patch = c->npatch > kMaxGreenPatches;
+    c->g = geometry_of(*opt, c->spectral_patch);
     const Geometry& g = c->g;
     auto cleanup = [&](int rc) {
         sc3d_destroy(c);
@@ -362,10 +405,25 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
         rc |= dmalloc(c, &c->ghat32, g.Ghat_elems);
     else
         rc |= dmalloc(c, &c->ghat64, g.Ghat_elems);
-    rc |= dmalloc(c, &c->tw64, kTwiddleElems);
-    rc |= dmalloc(c, &c->tw32, kTwiddleElems);
+    rc |= dmalloc(c, &c->tw64, g.tw_elems);
+    rc |= dmalloc(c, &c->tw32, g.tw_elems);
     rc |= dmalloc(c, &c->d_gj, 1);
+    if (c->spectral_patch) {
+        rc |= dmalloc(c, &p, int64_t(g.Dz) * g.Dy * g.Dx * rs);
+        c->Cbuf = p;
+        rc |= dmalloc(c, &p, int64_t(g.Dz) * g.Dy * g.Hx * 2 * rs);
+        c->chat = p;
+    }
     if (rc) return cleanup(rc);
+    if (c->spectral_patch &&
+        hipMemset(c->Cbuf, 0, size_t(g.Dz) * g.Dy * g.Dx * rs) != hipSuccess)
+        return cleanup(fail(SC3D_ERR_HIP, "sc3d_create: workspace initialisation failed"));
+    if (c->npatch > 0) {
+        if (int rp = dmalloc(c, &c->d_patch, c->npatch)) return cleanup(rp);
+        if (hipMemcpy(c->d_patch, patches.data(), patches.size() * sizeof(GreenPatch),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(SC3D_ERR_HIP, "sc3d_create: workspace initialisation failed"));
+    }
     c->gj.rho = c->rho;
     c->gj.S = c->S;
     c->gj.S2 = c->S2;
@@ -374,9 +432,10 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     c->gj.ghat64 = c->ghat64;
     c->gj.ghat = c->fp32 ? static_cast<void*>(c->ghat32) : static_cast<void*>(c->ghat64);
     c->gj.dom = c->dom;
+    c->gj.chat = c->chat;
     std::vector<double2> t64;
     std::vector<float2> t32;
-    make_twiddles(t64, t32);
+    make_twiddles(g, t64, t32);
     if (hipMemcpy(c->tw64, t64.data(), t64.size() * sizeof(double2), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(c->tw32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice) !=

@@ -47,6 +47,11 @@ struct sc3d_ctx {
     double* M = nullptr;      // moment partials (diagnostics)
     int64_t M_cap = 0;
     unsigned long long* tails = nullptr;  // tail counts (diagnostics)
+    int npatch = 0;                 // reference g2 patches of this shape
+    sc3d::GreenPatch* d_patch = nullptr;
+    bool spectral_patch = false;    // npatch > kMaxGreenPatches: correction as chat
+    void* Cbuf = nullptr;           // R[Dz][Dy][Dx] g2 correction (spectral_patch)
+    void* chat = nullptr;           // its spectrum, cplx<R>[Dz][Dy][Hx]
 };
 
 namespace sc3d {
@@ -106,6 +111,19 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
 // over the device table `table` (events ev[4..6] when timing).
 void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream_t st,
                   hipEvent_t* ev);
+
+// The points where the reference's g2 differs from the symmetric one at
+// this shape (sc3d_quirk.hip); returns their number.
+int green_patches(int green_fn, const int shape[3], std::vector<GreenPatch>& out);
+
+// Adds the patches' convolution with rho: to phi (FP64 form, before the
+// field) or, differenced, to E (precision form, after the field passes);
+// nothing for a spectral_patch context.
+void launch_green_patches(const sc3d_ctx* c, hipStream_t st);
+
+// spectral_patch contexts: the step's patch values into Cbuf and their
+// spectrum into chat (where h changed), after the Green's spectrum.
+void launch_green_chat(const sc3d_ctx* c, hipStream_t st);
 
 // The kick (with the fused post-kick maps: c->post, or the schedule's).
 void enqueue_kick(sc3d_ctx* c, const sc3d_bunch* b, const Sched& sc, hipStream_t st);

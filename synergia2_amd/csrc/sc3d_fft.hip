@@ -27,6 +27,7 @@
 // passes and stored as that octant only.
 #include <type_traits>
 
+#include "sc3d_green.h"
 #include "sc3d_internal.h"
 
 namespace sc3d {
@@ -43,10 +44,8 @@ template <int L> struct LdsLine {
     static __device__ __forceinline__ int at(int line, int p) { return line * stride + pad(p); }
 };
 
-bool fft_length_supported(int L)
-{
-    return L >= 8 && L <= 512 && (L & (L - 1)) == 0;
-}
+bool tuned_row_length(int gx) { return gx >= 8 && gx <= 256 && (gx & (gx - 1)) == 0; }
+bool tuned_col_length(int L) { return L >= 8 && L <= 512 && (L & (L - 1)) == 0; }
 
 // Lines per block: at least 8 so that strided passes move >=128 B runs, and
 // at least 1024 elements so every thread owns one radix-4 butterfly.
@@ -711,42 +710,12 @@ __global__ __launch_bounds__(kThreads) void k_x_field(
 }
 
 // ---------------------------------------------------------------------------
-// Green's functions (space_charge_3d_open_hockney.cc:136-348).  Value at the
-// doubled-grid point (X, Y, Z) is f(x, y, z) with x = min(X, Dx-X)*hx etc.,
-// exactly the value the reference's mirror writes leave there.
+// Green's spectrum (space_charge_3d_open_hockney.cc:136-348).  The value at
+// the doubled-grid point (X, Y, Z) is f(x, y, z) with x = min(X, Dx-X)*hx
+// etc. (sc3d_green.h), the value the reference's mirror writes leave there
+// wherever its index decomposition is exact (sc3d_quirk.hip patches the
+// few points of the shapes where it is not).
 // ---------------------------------------------------------------------------
-__device__ double green_linear(double x, double y, double z, double hz, double G000)
-{
-    double rr = x * x + y * y;
-    double s_rr_0 = sqrt(rr + z * z);
-    double s_rr_n = sqrt(rr + (z - hz) * (z - hz));
-    double s_rr_p = sqrt(rr + (z + hz) * (z + hz));
-    double G = 2.0 * s_rr_0 - s_rr_n - s_rr_p;
-    double T1, T2, r1, r2;
-    const double epsz = 1e-12 * hz;
-    // z >= 0 here, so of the reference's branches (:294-326) only these three
-    // are reachable; the z < -hz and |z + hz| < epsz ones never are.
-    if (fabs(z) < epsz) {
-        if (fabs(x) + fabs(y) < 2. * epsz) {
-            G += hz * G000;
-        } else {
-            r1 = (sqrt(hz * hz + rr) + hz) / sqrt(rr);
-            G += 2.0 * hz * log(r1);
-        }
-    } else if (fabs(z - hz) < epsz) {
-        r1 = (s_rr_p + z + hz) / (s_rr_0 + z);
-        T1 = (hz + z) * log(r1);
-        G += T1;
-    } else if (z > hz) {
-        r1 = (s_rr_0 + z) / (s_rr_n + z - hz);
-        T1 = (hz - z) * log(r1);
-        r2 = (s_rr_p + z + hz) / (s_rr_0 + z);
-        T2 = (hz + z) * log(r2);
-        G += T1 + T2;
-    }
-    return G;
-}
-
 // The spectrum is transformed in T: FP64 in the FP64 form, FP32 in the
 // precision form (whose convolution reads it in FP32).  The function values
 // are always evaluated in FP64 (2 s0 - sn - sp cancels) and rounded once.
@@ -778,14 +747,7 @@ __global__ __launch_bounds__(kThreads) void k_green_x(
     const int nrows = (gz + 1) * (gy + 1);
     const int r0 = blockIdx.x * NL;
     const double hx = dom->h[0], hy = dom->h[1], hz = dom->h[2];
-    double G000;
-    if (green_fn == SC3D_GREEN_LINEAR) {
-        double rr = hx * hx + hy * hy;
-        double r1 = sqrt(hx * hx + hy * hy + hz * hz);
-        G000 = (2.0 / rr) * (hz * r1 + rr * log((hz + r1) / sqrt(rr)) - hz * hz);
-    } else {
-        G000 = 2.8 / fmin(hx, fmin(hy, hz));
-    }
+    const double G000 = green_g000(green_fn, hx, hy, hz);
     for (int e = threadIdx.x; e < NL * (M + 1); e += kThreads) {
         int line = e / (M + 1), X = e % (M + 1);
         int r = r0 + line;
@@ -793,11 +755,7 @@ __global__ __launch_bounds__(kThreads) void k_green_x(
         if (r < nrows) {
             int Z = r / (gy + 1), Y = r - Z * (gy + 1);
             double x = X * hx, y = Y * hy, z = Z * hz;
-            if (green_fn == SC3D_GREEN_LINEAR) {
-                v = green_linear(x, y, z, hz, G000);
-            } else {
-                v = (X == 0 && Y == 0 && Z == 0) ? G000 : 1.0 / sqrt(x * x + y * y + z * z);
-            }
+            v = green_value(green_fn, x, y, z, hz, G000);
         }
         row[e] = T(v);
     }
@@ -894,7 +852,7 @@ void dispatch_len(int L, A&&... a)
     case 128: F<128>::run(a...); break;
     case 256: F<256>::run(a...); break;
     case 512: F<512>::run(a...); break;
-    default: break;
+    default: break;  // the launchers route other lengths to the generic passes
     }
 }
 
@@ -945,9 +903,12 @@ template <typename R>
 void launch_green(const Geometry& g, int green_fn, const GridJob* jobs, int nj,
                   const typename cplx<R>::type* tw, hipStream_t st)
 {
-    dispatch_len<Green<R>::template X>(g.gx, g, green_fn, jobs, nj, tw, st);
-    dispatch_len<Green<R>::template Y>(g.Dy, g, jobs, nj, tw, st);
-    dispatch_len<Green<R>::template Z>(g.Dz, g, jobs, nj, tw, st);
+    if (g.tuned[0]) dispatch_len<Green<R>::template X>(g.gx, g, green_fn, jobs, nj, tw, st);
+    else any_green_x<R>(g, green_fn, jobs, nj, tw, st);
+    if (g.tuned[1]) dispatch_len<Green<R>::template Y>(g.Dy, g, jobs, nj, tw, st);
+    else any_green_cols<R>(g, 1, jobs, nj, tw, st);
+    if (g.tuned[2]) dispatch_len<Green<R>::template Z>(g.Dz, g, jobs, nj, tw, st);
+    else any_green_cols<R>(g, 2, jobs, nj, tw, st);
 }
 
 template <typename R> struct SolveX {
@@ -1044,22 +1005,32 @@ template <typename R>
 void launch_solve(const Geometry& g, const GridJob* jobs, int nj,
                   const typename cplx<R>::type* tw, hipStream_t st)
 {
-    dispatch_len<SolveX<R>::template Fwd>(g.gx, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template ZConv>(g.Dz, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template YInv>(g.Dy, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template Inv>(g.gx, g, jobs, nj, tw, st);
+    if (g.tuned[0]) dispatch_len<SolveX<R>::template Fwd>(g.gx, g, jobs, nj, tw, st);
+    else any_x_pass<R>(g, false, jobs, nj, tw, st);
+    if (g.tuned[1]) dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, jobs, nj, tw, st);
+    else any_cols<R>(g, 0, jobs, nj, tw, st);
+    if (g.tuned[2]) dispatch_len<SolveX<R>::template ZConv>(g.Dz, g, jobs, nj, tw, st);
+    else any_cols<R>(g, 2, jobs, nj, tw, st);
+    if (g.tuned[1]) dispatch_len<SolveX<R>::template YInv>(g.Dy, g, jobs, nj, tw, st);
+    else any_cols<R>(g, 1, jobs, nj, tw, st);
+    if (g.tuned[0]) dispatch_len<SolveX<R>::template Inv>(g.gx, g, jobs, nj, tw, st);
+    else any_x_pass<R>(g, true, jobs, nj, tw, st);
 }
 
 template <typename R>
 void launch_solve_field(const Geometry& g, const GridJob* jobs, int nj,
                         const typename cplx<R>::type* tw, hipStream_t st)
 {
-    dispatch_len<SolveX<R>::template Fwd>(g.gx, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template ZField>(g.Dz, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template YField>(g.Dy, g, jobs, nj, tw, st);
-    dispatch_len<SolveX<R>::template XField>(g.gx, g, jobs, nj, tw, st);
+    if (g.tuned[0]) dispatch_len<SolveX<R>::template Fwd>(g.gx, g, jobs, nj, tw, st);
+    else any_x_pass<R>(g, false, jobs, nj, tw, st);
+    if (g.tuned[1]) dispatch_len<SolveX<R>::template YFwd>(g.Dy, g, jobs, nj, tw, st);
+    else any_cols<R>(g, 0, jobs, nj, tw, st);
+    if (g.tuned[2]) dispatch_len<SolveX<R>::template ZField>(g.Dz, g, jobs, nj, tw, st);
+    else any_field<R>(g, 0, jobs, nj, tw, st);
+    if (g.tuned[1]) dispatch_len<SolveX<R>::template YField>(g.Dy, g, jobs, nj, tw, st);
+    else any_field<R>(g, 1, jobs, nj, tw, st);
+    if (g.tuned[0]) dispatch_len<SolveX<R>::template XField>(g.gx, g, jobs, nj, tw, st);
+    else any_field<R>(g, 2, jobs, nj, tw, st);
 }
 
 template void launch_green<double>(const Geometry&, int, const GridJob*, int, const double2*,

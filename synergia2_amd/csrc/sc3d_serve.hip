@@ -536,6 +536,10 @@ int sc3d_server_iterate(sc3d_server* s, int32_t* n_resident, int32_t* n_waiting)
         if (j->steps_done == j->spec.steps)
             if (int rc = finish(s, j)) return rc;
     }
+    // nothing left to run: the finished jobs (results already copied out)
+    // release their memory now, not at the next call
+    if (s->active() == 0 && !s->resident.empty())
+        if (int rc = rebuild(s)) return rc;
     if (n_resident) *n_resident = int32_t(s->active());
     if (n_waiting) *n_waiting = int32_t(s->waiting.size());
     return SC3D_OK;

@@ -378,9 +378,11 @@ def test_errors_and_empty(gpu):
     op3 = sc.SpaceCharge3dOpenHockney(make_opts((32, 32, 64), fixed=((0, 0, 0), (1, 1, 1))))
     op3.apply_bunch(e, 1e-9)
     assert np.all(op3.debug(sc.DBG_RHO) == 0)
-    # unsupported shape
-    with pytest.raises(sc.Sc3dError):
-        sc.SpaceCharge3dOpenHockney(sc.default_options(30, 32, 64))
+    # unsupported shapes: an axis below 4 or above 512 (doubled length 1024)
+    for shape in ((2, 32, 64), (32, 600, 8)):
+        with pytest.raises(sc.Sc3dError) as ei:
+            sc.SpaceCharge3dOpenHockney(sc.default_options(*shape))
+        assert ei.value.code == sc.SC3D_ERR_ARG and "[4, 512]" in str(ei.value)
 
 
 def test_populate_bit_exact_vs_oracle(gpu):
@@ -684,3 +686,78 @@ def test_fixture_bunch_1k(gpu):
         assert rel_l2(op.debug(tap), fx[name]) <= 1e-12, name
     pg, mg = b.host()
     check_kicked(pg, mg, fx["parts_out"], parts)
+
+
+@pytest.mark.parametrize("green,gname", [(sc.GREEN_LINEAR, "linear"),
+                                         (sc.GREEN_POINTLIKE, "pointlike")])
+@pytest.mark.parametrize("shape", [(16, 32, 16), (32, 16, 8)])
+def test_reference_g2_index_quirk_shapes(gpu, green, gname, shape):
+    """At the shapes where alg_g2_*'s float-reciprocal index decomposition
+    misplaces points (space_charge_3d_open_hockney.cc:170-176,272-278), the
+    oracle replays it (serial write order) and the device adds the patch
+    points' convolution: phi and E within 1e-12 of the oracle in the FP64
+    form, E within 2e-6 in the precision form (bar 1e-5), kicks within
+    1e-12."""
+    n = 30000
+    parts = gaussian_parts(n, (2e-3, 1e-4, 1e-3, 1e-4, 5e-3, 1e-4), seed=11)
+    masks = np.ones(n, np.uint8)
+    dt = 0.1 / (BETA * sc.C_LIGHT)
+    print(f"case quirk {shape} {gname}", flush=True)
+    rc, out = oracle_run(oracle_kw(shape, gname), parts, masks, 1e10, dt,
+                         want=("rho", "phi", "ex", "ey", "ez"))
+    assert rc == 0
+    op, b = gpu_run(make_opts(shape, green), parts, masks, 1e10, dt)
+    assert rel_l2(op.debug(sc.DBG_RHO), out["rho"]) <= 1e-13
+    assert rel_l2(op.debug(sc.DBG_PHI), out["phi"]) <= 1e-12
+    for tap, name in ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez")):
+        assert rel_l2(op.debug(tap), out[name]) <= 1e-12, name
+    pg, mg = b.host()
+    check_kicked(pg, mg, out["parts"], parts)
+    op32, _ = gpu_run(make_opts(shape, green, precision=sc.FP32_GRID), parts, masks, 1e10, dt)
+    errs = [rel_l2(op32.debug(t), out[nm]) for t, nm in
+            ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez"))]
+    print(f"  precision form field rel-L2 {errs}", flush=True)
+    assert max(errs) <= 2e-6
+
+
+@pytest.mark.parametrize("shape", [(24, 24, 48), (48, 48, 96), (48, 64, 128), (12, 20, 37),
+                                   (15, 16, 300), (320, 8, 8)])
+def test_any_shape_vs_oracle(gpu, shape):
+    """Shapes outside the tuned power-of-two transforms -- factors 3 and 5
+    (24x24x48, 48x48x96), a mix of generic x and tuned y, z passes
+    (48x64x128), a prime factor (Dz = 74 = 2 * 37), an odd gx with a 600-long
+    z axis (15x16x300) and gx above 256 (Dx = 640) -- against the oracle, whose
+    non-power-of-two transforms are direct DFTs pinned by numpy golden
+    vectors.  At gx = 48 the reference's g2 index decomposition misplaces
+    points in most rows of every plane (29376 / 63744 points), which the
+    device applies as a correction spectrum.  Green's spectrum <= 1e-13 of
+    the full FFT of the reference's g2 (where it is symmetric),
+    rho <= 1e-13, phi and E <= 1e-12, kicks <= 1e-12; precision form field
+    <= 2e-6 (bar 1e-5)."""
+    n = 30000
+    parts = gaussian_parts(n, (2e-3, 1e-4, 1.5e-3, 1e-4, 5e-3, 1e-4), seed=12)
+    masks = np.ones(n, np.uint8)
+    dt = 0.1 / (BETA * sc.C_LIGHT)
+    print(f"case any shape {shape}", flush=True)
+    rc, out = oracle_run(oracle_kw(shape), parts, masks, 1e10, dt,
+                         want=("rho", "phi", "ex", "ey", "ez", "ghat_full"))
+    assert rc == 0, O.last_error()
+    op, b = gpu_run(make_opts(shape), parts, masks, 1e10, dt)
+    gx, gy, gz = shape
+    if len(sc.green_patches(sc.GREEN_LINEAR, shape)) == 0:
+        # (at 48x48x96 and 48x64x128 the reference's g2 has misplaced points,
+        # which the device adds as a separate correction spectrum)
+        full = out["ghat_full"].view(np.complex128).reshape(2 * gz, 2 * gy, gx + 1)
+        got = op.debug(sc.DBG_GHAT).reshape(gz + 1, gy + 1, gx + 1)
+        assert rel_l2(got, full[: gz + 1, : gy + 1, :].real) <= 1e-13
+    assert rel_l2(op.debug(sc.DBG_RHO), out["rho"]) <= 1e-13
+    assert rel_l2(op.debug(sc.DBG_PHI), out["phi"]) <= 1e-12
+    for tap, name in ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez")):
+        assert rel_l2(op.debug(tap), out[name]) <= 1e-12, name
+    pg, mg = b.host()
+    check_kicked(pg, mg, out["parts"], parts)
+    op32, _ = gpu_run(make_opts(shape, precision=sc.FP32_GRID), parts, masks, 1e10, dt)
+    errs = [rel_l2(op32.debug(t), out[nm]) for t, nm in
+            ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez"))]
+    print(f"  precision form field rel-L2 {errs}", flush=True)
+    assert max(errs) <= 2e-6

@@ -12,14 +12,14 @@ import oracle as O
 GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_golden.npz"))
 
 
-@pytest.mark.parametrize("n", [8, 16, 12, 64])
+@pytest.mark.parametrize("n", [8, 16, 12, 64, 30, 74])
 def test_fft1d_golden(n):
     x = GOLD[f"c1d_{n}_in"]
     np.testing.assert_allclose(O.fft1d(x, -1), GOLD[f"c1d_{n}_fwd"], rtol=0, atol=1e-12 * n)
     np.testing.assert_allclose(O.fft1d(x, +1), GOLD[f"c1d_{n}_inv"], rtol=0, atol=1e-12 * n)
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 16), (16, 8, 8), (8, 4, 8)])
+@pytest.mark.parametrize("dims", [(8, 8, 16), (16, 8, 8), (8, 4, 8), (12, 10, 6), (10, 6, 14)])
 def test_fft3d_r2c_c2r_golden(dims):
     nx, ny, nz = dims
     r = GOLD[f"r2c_{nx}x{ny}x{nz}_in"]

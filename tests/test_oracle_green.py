@@ -84,3 +84,57 @@ def test_reference_index_quirk():
         assert reference_decomposition_exact(shape[0], shape[1], 2 * shape[2])
         assert reference_decomposition_exact(shape[0], shape[1], shape[2] + 1)
     assert not reference_decomposition_exact(16, 32, 32)
+
+
+def closed_form(green, x, y, z, h):
+    if green == "linear":
+        return linear_closed_form(x, y, z, *h)
+    return 2.8 / min(h) if x == y == z == 0 else 1.0 / math.sqrt(x * x + y * y + z * z)
+
+
+@pytest.mark.parametrize("green", ["linear", "pointlike"])
+@pytest.mark.parametrize("shape", [(16, 32, 16), (32, 16, 8)])
+def test_green_patches_reproduce_reference_g2(green, shape):
+    """At the shapes where alg_g2_*'s float-reciprocal decomposition
+    misplaces indices, the library's patch list (host side of the device
+    path) names exactly the points where the oracle's g2 -- every index
+    replayed in the reference's serial order -- differs from the symmetric
+    closed form, and the value the oracle holds there: the closed form at
+    the misplaced writer's coordinates, or 0 where nothing is written."""
+    import synergia2_amd as sc
+    gfn = sc.GREEN_LINEAR if green == "linear" else sc.GREEN_POINTLIKE
+    patches = sc.green_patches(gfn, shape)
+    assert 0 < len(patches) <= 512
+    h = (1.3e-4, 2.1e-4, 7.0e-4)
+    g2 = O.green_fn2(green, shape, h)
+    D = [2 * shape[0], 2 * shape[1], 2 * shape[2]]
+    targets = {}
+    for t0, t1, t2, s0, s1, s2 in patches:
+        targets[(t0, t1, t2)] = None if s0 < 0 else (s0, s1, s2)
+    assert len(targets) == len(patches)
+    for Z in range(D[2]):
+        for Y in range(D[1]):
+            for X in range(D[0]):
+                if (Z, Y, X) in targets:
+                    src = targets[(Z, Y, X)]
+                    if src is None:
+                        assert g2[Z, Y, X] == 0.0
+                    else:
+                        iz, iy, ix = src
+                        z = (D[2] - iz if (green == "linear" and iz > shape[2]) else iz) * h[2]
+                        ref = closed_form(green, ix * h[0], iy * h[1], z, h)
+                        assert g2[Z, Y, X] == pytest.approx(ref, rel=1e-14, abs=0)
+                        sym = closed_form(green, min(X, D[0] - X) * h[0],
+                                          min(Y, D[1] - Y) * h[1], min(Z, D[2] - Z) * h[2], h)
+                        assert abs(g2[Z, Y, X] / sym - 1) > 1e-6  # a real difference
+                else:
+                    sym = closed_form(green, min(X, D[0] - X) * h[0], min(Y, D[1] - Y) * h[1],
+                                      min(Z, D[2] - Z) * h[2], h)
+                    assert g2[Z, Y, X] == pytest.approx(sym, rel=1e-14, abs=0), (Z, Y, X)
+
+
+def test_green_patches_none_at_configured_shapes():
+    import synergia2_amd as sc
+    for shape in ((32, 32, 64), (64, 64, 128), (128, 128, 256), (8, 8, 16), (64, 64, 64)):
+        for gfn in (sc.GREEN_LINEAR, sc.GREEN_POINTLIKE):
+            assert len(sc.green_patches(gfn, shape)) == 0
