@@ -232,6 +232,17 @@ int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n,
                            const double chol[21], uint64_t seed, int64_t id0,
                            void* stream);
 
+/* sc3d_moments without a context and without allocating: no reference
+ * counterpart; for callers that cannot call hipMalloc (code under stream
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * chunk partials live in a fixed device array of the library (2.6 MiB per
+ * GPU, held until the process ends), so at most 2^26 active particles
+ * (larger: SC3D_ERR_SIZE, use sc3d_moments).  Results equal sc3d_moments'
+ * bit for bit.  Calls that overlap on one GPU (other streams or threads)
+ * race on that array: serialise them. */
+int sc3d_moments_noalloc(const sc3d_bunch* bunch, double mean[6], double mom2[36],
+                         void* stream);
+
 /* Host only (no device call).  The points of the doubled-grid Green's
  * function where the reference's flat-index decomposition in alg_g2_linear /
  * alg_g2_pointlike (space_charge_3d_open_hockney.cc:170-176,272-278) leaves

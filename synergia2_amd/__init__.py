@@ -125,6 +125,7 @@ EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
            "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
            "sc3d_moments", "sc3d_tail_fraction", "sc3d_green_patches",
+           "sc3d_moments_noalloc",
            # served form (include/sc3d_serve.h)
            "sc3d_server_options_default", "sc3d_server_create", "sc3d_server_submit",
            "sc3d_server_iterate", "sc3d_server_job", "sc3d_server_job_particles",
@@ -169,6 +170,9 @@ def load(path=None):
     L.sc3d_green_patches.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.c_int32]
     L.sc3d_green_patches.restype = C.c_int
+    L.sc3d_moments_noalloc.argtypes = [C.POINTER(BunchDesc), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.c_void_p]
+    L.sc3d_moments_noalloc.restype = C.c_int
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
                  "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
                  "sc3d_get_timing", "sc3d_set_post_kick_maps", "sc3d_moments",

@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SC3D_ARCH", "gfx950")
 
 HIP_SOURCES = ["sc3d_fft.hip", "sc3d_fft_any.hip", "sc3d_particles.hip", "sc3d_maps.hip", "sc3d_capi.hip",
-               "sc3d_serve.hip", "sc3d_quirk.hip"]
+               "sc3d_serve.hip", "sc3d_quirk.hip", "sc3d_noalloc.hip"]
 LIB = os.path.join(HERE, "libsc3d_hip.so")
 HOST_LIB = os.path.join(HERE, "libsc3d_host.so")
 
