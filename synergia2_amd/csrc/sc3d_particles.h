@@ -64,11 +64,11 @@ void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, co
                     double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st);
 template <typename R>
 void launch_fill(R* p, int64_t n, hipStream_t st);
-// Deposit: local (k_deposit_local): windows of the bunch storage, sorted by
-// cell in LDS (the storage is cell-sorted at every reorder); else
-// (k_deposit): through the permutation `perm` of this step's sort.  FP32:
-// unit weights, compensated run sums; then k_rho_peak records the density
-// peak in the domain record.
+// Deposit: local (k_deposit_tile): windows of the bunch storage through an
+// LDS node tile of each window's cell box (the storage is cell-sorted at
+// every reorder); else (k_deposit): through the permutation `perm` of this
+// step's sort, compensated run sums in FP32.  FP32: unit weights; then
+// k_rho_peak records the density peak in the domain record.
 template <typename R>
 void launch_deposit(const double* parts, int64_t stride, const uint8_t* masks,
                     const int32_t* perm, bool local, int64_t n, int gx, int gy, int gz,

@@ -710,11 +710,10 @@ __global__ void k_fill(R* __restrict__ p, int64_t n, R v)
 // written only if 0 < idx < g-1 in all three axes (ingrid, :21-30).
 // Target: the G octant of rho2 (zyx, gx fastest).
 //
-// The eight corner weights are summed over runs of equal cell keys before
-// any global atomic.  Every lane takes K consecutive particles of a
-// cell-sorted order (a window of the bunch storage sorted by cell in LDS,
-// k_deposit_local; or, when the storage is never reordered, this step's
-// sort permutation, k_deposit) and sums its own runs in order; a run that
+// Through the sort permutation (k_deposit, used when the storage is never
+// reordered): the eight corner weights are summed over runs of equal cell
+// keys before any global atomic.  Every lane takes K consecutive particles
+// of the cell-sorted order and sums its own runs in order; a run that
 // starts and ends inside the lane is emitted at once.  The lanes' trailing runs are
 // then summed across lanes by one segmented inclusive scan (a segment
 // continues through lanes that hold a single run of the same key), and the
@@ -989,73 +988,172 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
     deposit_runs<R, K>(key, ox, oy, oz, dom->w0, AtomicRuns<R>{rho, gx, gy, gz});
 }
 
-// Deposit in windows of the bunch storage.  The storage is cell-sorted at
-// every reorder (sort_period applies) and particles move a few cells per
-// step, so a window of kWindow consecutive slots stays a compact group of
-// cells: the block loads its window coalesced, sorts (key, slot) by cell key
-// in LDS (stable LSD radix over the bits of key - min key of the window,
-// <= 8-bit digits, wave-private ranking as in the global scatter; skipped
-// when the window is already in cell order), and sums it in that order with
-// the lane runs (4 particles per lane; the wavefront scan moves partial sums
-// between lanes with DPP).  The runs go into an LDS list of the window (key
-// order, over the sort buffers; plain stores, atomics only for the runs
-// across a wavefront edge); there each cell adds its four +x corners
-// together with the -x corners of the next cell when that cell is its row
-// neighbour, so shared nodes take one atomic instead of two (a window with
-// more runs than the list holds adds its runs directly).  A cell whose
-// particles fall in several windows is added once per window.  Measured at
-// 4e6 / 1.6e7 particles (deposit phase, us, sort_period 8, before the run
-// list): windows of 1024 with the offsets held in LDS 158 / 590; 1024
-// re-reading the coordinates 141 / 493; 2048 re-reading 137 / 530; 4096
-// re-reading 166 / 726.  The run list: 104 / 455.
-constexpr int kWindow = 1024;
+// Deposit in windows of the bunch storage through an LDS node tile.  The
+// storage is cell-sorted at every reorder (sort_period applies) and
+// particles move a few cells per step, so kTileWindow consecutive slots stay
+// a compact group of cells.  A block takes one window, each thread four
+// consecutive slots; the block reduces the window's cell bounding box, clears
+// that box of nodes in LDS (up to kTileNodes FP64 nodes), and every particle
+// adds its eight corner weights there with LDS FP64 atomics -- the corners of
+// a thread's consecutive particles in one cell summed in registers first,
+// the corner order permuted by lane so that the lanes of one cell hit eight
+// different nodes at a time.  The block then adds the box's non-zero
+// interior nodes to rho with global atomics (no sort, no scan, no run list).
+// A window whose box does not fit gives each wavefront a quarter of the
+// tile for the box of its own 256 particles (no block barrier: LDS
+// operations of one wavefront complete in order); a wavefront whose box does
+// not fit either sums its corners in a direct-mapped FP64 cache of nodes in
+// its quarter (a crowded node of a scattered storage then takes one add per
+// wavefront), the rest going to rho directly.  Those wide
+// windows lie at the two ends of the cell order (the tails of the bunch), so
+// the block order is rotated to start the last sixteenth of the windows
+// first and keep them off the end of the kernel.  The tile accumulates in
+// FP64 for both precisions (the precision form's unit weights are rounded
+// to FP32 once, at the global add).
+// Measured against the windowed kernel it replaces (LDS radix sort of each
+// window, lane runs, DPP segmented scan, LDS run list; PMC: 465 VALU
+// instructions per particle), deposit phase per step over sort periods of
+// 12: 96.1 -> 54.9 us at 4e6 particles FP64 (64x64x128), 333.5 -> 214.7 us
+// at 1.6e7 FP32 (128x128x256).  Variants measured on the way (us at 4e6,
+// over a sort period): one particle per lane-stride instead of four
+// consecutive per thread 81-99; tiles of 2048 / 3072 nodes 93-105 / 83-93;
+// 512-thread blocks (6144 / 8192 nodes) 61-102 / 60-75; without the
+// per-wavefront fallback and the block-order rotation 68-78; the corner sums
+// picked by an indexed load (through scratch) 107-134.
+constexpr int kTileThreads = 256;
+constexpr int kTilePer = 4;                             // consecutive slots per thread
+constexpr int kTileWindow = kTileThreads * kTilePer;    // slots per block
+constexpr int kTileNodes = 4096;                        // FP64 nodes of the LDS tile (32 KiB)
+constexpr int kTileWaves = kTileThreads / 64;
 
-// The kernel is latency-bound, so blocks per CU matter; both the run list
-// (LDS) and the registers are sized for them.  FP64: a list of 384 runs
-// (31 KiB of LDS) and 96 VGPRs (8 bytes spilled) give 5 blocks per CU; FP32:
-// 512 runs (24 KiB) and 80 VGPRs (76 bytes spilled) give 6.  Measured (us per
-// deposit, 4e6 FP64 / 1.6e7 FP32) against 512 runs and 4 / 5 waves per SIMD:
-// 96.7 vs 100.7 / 334 vs 359; FP32 with the 384-run list 364.  (Before the
-// run list's plain stores, 5 FP64 and 6 FP32 waves measured slower: their
-// spills were larger.)
-template <typename R> struct DepositShape {
-    static constexpr bool fp64 = sizeof(R) == sizeof(double);
-    static constexpr int run_cap = fp64 ? 384 : 512;  // runs held in LDS (more: direct atomics)
-    static constexpr int waves = fp64 ? 5 : 6;        // per SIMD
-};
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o *= 2) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o *= 2) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Corner weights of one particle in the reference's order, w0*wx*wy*wz left
+// to right (deposit.cc:366-391); corner c = (x bit 0, y bit 1, z bit 2).
+__device__ __forceinline__ void tile_weights(double w0, double ox, double oy, double oz,
+                                             double (&w)[8])
+{
+    const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const double wx = (c & 1) ? ox : aox, wy = (c & 2) ? oy : aoy, wz = (c & 4) ? oz : aoz;
+        w[c] = w0 * wx * wy * wz;
+    }
+}
+
+// Adds the eight corner sums of the cell at tile offset b0 (box of nx x ny
+// x .. nodes): at step k corner k ^ (lane mod 8), so the lanes of one cell
+// hit eight different nodes at a time.  The sums are permuted by three
+// conditional register swaps (an indexed pick would go through scratch).
+__device__ __forceinline__ void tile_add(double* tile, int b0, int nx, int ny, int lane,
+                                         const double (&w)[8])
+{
+    double v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = w[c];
+    const int r = lane & 7;
+#pragma unroll
+    for (int bit = 1; bit < 8; bit <<= 1) {
+        const bool sw = r & bit;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (!(c & bit)) {
+                const double a = v[c], b = v[c | bit];
+                v[c] = sw ? b : a;
+                v[c | bit] = sw ? a : b;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int c = k ^ r;  // v[k] holds corner c
+        const int off = ((c >> 2) * ny + ((c >> 1) & 1)) * nx + (c & 1);
+        unsafeAtomicAdd(&tile[b0 + off], v[k]);
+    }
+}
+
+// This thread's particles into a tile whose box starts at (bx, by, bz):
+// consecutive particles of one cell summed first.
+__device__ __forceinline__ void tile_deposit(double* tile, int bx, int by, int bz, int nx, int ny,
+                                             int lane, double w0, const int (&ix)[kTilePer],
+                                             const int (&iy)[kTilePer], const int (&iz)[kTilePer],
+                                             const bool (&ok)[kTilePer],
+                                             const double (&ox)[kTilePer],
+                                             const double (&oy)[kTilePer],
+                                             const double (&oz)[kTilePer])
+{
+    double s[8];
+    int cur = -1;
+#pragma unroll
+    for (int q = 0; q < kTilePer; ++q) {
+        if (!ok[q]) continue;
+        const int b0 = ((iz[q] - bz) * ny + (iy[q] - by)) * nx + (ix[q] - bx);
+        double w[8];
+        tile_weights(w0, ox[q], oy[q], oz[q], w);
+        if (b0 == cur) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s[c] += w[c];
+        } else {
+            if (cur >= 0) tile_add(tile, cur, nx, ny, lane, s);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s[c] = w[c];
+            cur = b0;
+        }
+    }
+    if (cur >= 0) tile_add(tile, cur, nx, ny, lane, s);
+}
+
+// The non-zero interior nodes of a tile (box at (bx, by, bz), nx x ny x ..,
+// `nodes` in all) into rho, node e of the thread's share first..step.
 template <typename R>
-__global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(DepositShape<R>::waves))) void k_deposit_local(
+__device__ __forceinline__ void tile_flush(const double* tile, int nodes, int first, int step,
+                                           int bx, int by, int bz, int nx, int ny, int gx, int gy,
+                                           int gz, R* __restrict__ rho)
+{
+    for (int e = first; e < nodes; e += step) {
+        const double v = tile[e];
+        if (v == 0.0) continue;
+        const int cx = bx + e % nx, r = e / nx;
+        const int cy = by + r % ny, cz = bz + r / ny;
+        if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1)
+            unsafeAtomicAdd(rho + (int64_t(cz) * gy + cy) * gx + cx, R(v));
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kTileThreads) void k_deposit_tile(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
     int64_t n, int gx, int gy, int gz, const DomainRec* __restrict__ dom, R* __restrict__ rho)
 {
-    constexpr int W = kWindow, PER = kWindow / kThreads, kWaves = kThreads / 64;
-    constexpr int kRunCap = DepositShape<R>::run_cap;
-    // the sort buffers and, after the sort, the window's run list share LDS
-    constexpr int kSortBytes = 4 * W * int(sizeof(int32_t));
-    constexpr int kListBytes = kRunCap * int(sizeof(int32_t) + 8 * sizeof(R));
-    constexpr int kBytes = kSortBytes > kListBytes ? kSortBytes : kListBytes;
-    __shared__ __align__(16) unsigned char lds[kBytes];
-    int32_t(*skey)[W] = reinterpret_cast<int32_t(*)[W]>(lds);
-    int32_t(*sslot)[W] = reinterpret_cast<int32_t(*)[W]>(lds + 2 * W * sizeof(int32_t));
-    R(*rw)[8] = reinterpret_cast<R(*)[8]>(lds);
-    int32_t* rkey = reinterpret_cast<int32_t*>(lds + kRunCap * 8 * sizeof(R));
-    __shared__ int32_t wave_off[kWaves][kMaxBuckets];
-    __shared__ int32_t start_d[kMaxBuckets];
-    __shared__ int32_t wsum[kWaves];
-    __shared__ int32_t edge_run[kWaves];
+    __shared__ double tile[kTileNodes];
+    __shared__ int box[kTileWaves][6];
     if (dom->error) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
-    // off-grid, masked and absent slots sort last under G-1, which no cell
-    // on the grid has (ix <= gx-2)
-    const int sentinel = gx * gy * gz - 1;
-    const int64_t base = int64_t(blockIdx.x) * W;
+    // precision form: unit weights (w0 and the unit-peak scale return at the kick)
+    const double w0 = sizeof(R) == sizeof(double) ? dom->w0 : 1.0;
+    // the last sixteenth of the windows (one tail of the bunch) first
+    const int nb = int(gridDim.x), tail = nb / 16;
+    const int blk = (int(blockIdx.x) + nb - tail) % nb;
+    const int64_t base = int64_t(blk) * kTileWindow + int64_t(t) * kTilePer;
+    int ix[kTilePer], iy[kTilePer], iz[kTilePer];
+    double ox[kTilePer], oy[kTilePer], oz[kTilePer];
+    bool ok[kTilePer];
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {-1, -1, -1};
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int sl = q * kThreads + t;
-        const int64_t j = base + sl;
+    for (int q = 0; q < kTilePer; ++q) {
+        const int64_t j = base + q;
         double x = 0.0, y = 0.0, z = 0.0;
         bool m = false;
         if (j < n) {
@@ -1064,199 +1162,106 @@ __attribute__((amdgpu_waves_per_eu(DepositShape<R>::waves))) void k_deposit_loca
             y = parts[j + 2 * stride];
             z = parts[j + 4 * stride];
         }
-        double ox, oy, oz;
-        int key = sentinel;
-        if (m) {
-            const int ix = cell_index(x, lx, ihx, ox);
-            const int iy = cell_index(y, ly, ihy, oy);
-            const int iz = cell_index(z, lz, ihz, oz);
-            if (ix >= 0 && ix <= gx - 2 && iy >= 0 && iy <= gy - 2 && iz >= 0 && iz <= gz - 2)
-                key = (iz * gy + iy) * gx + ix;
+        ix[q] = cell_index(x, lx, ihx, ox[q]);
+        iy[q] = cell_index(y, ly, ihy, oy[q]);
+        iz[q] = cell_index(z, lz, ihz, oz[q]);
+        // a particle outside [0, g-2] in any axis has no interior corner
+        ok[q] = m && ix[q] >= 0 && ix[q] <= gx - 2 && iy[q] >= 0 && iy[q] <= gy - 2 &&
+                iz[q] >= 0 && iz[q] <= gz - 2;
+        if (ok[q]) {
+            lo[0] = min(lo[0], ix[q]); hi[0] = max(hi[0], ix[q]);
+            lo[1] = min(lo[1], iy[q]); hi[1] = max(hi[1], iy[q]);
+            lo[2] = min(lo[2], iz[q]); hi[2] = max(hi[2], iz[q]);
         }
-        skey[0][sl] = key;
-        sslot[0][sl] = sl;
-    }
-    // key range of the window (the sentinel excluded); a window already in
-    // cell order (right after a reorder, or where nothing crossed a cell
-    // boundary) is not sorted again
-    int kmin = sentinel, kmax = 0;
-    bool desc = false;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int sl = q * kThreads + t;
-        const int k = skey[0][sl];
-        if (k != sentinel) {
-            kmin = min(kmin, k);
-            kmax = max(kmax, k);
-        }
-        if (sl > 0 && skey[0][sl - 1] > k) desc = true;
     }
 #pragma unroll
-    for (int w = 1; w < 64; w *= 2) {
-        kmin = min(kmin, __shfl_xor(kmin, w, 64));
-        kmax = max(kmax, __shfl_xor(kmax, w, 64));
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(lo[a]);
+        hi[a] = wave_max(hi[a]);
     }
-    if (lane == 0) {
-        wave_off[0][wave] = kmin;
-        wave_off[1][wave] = kmax;
-    }
-    desc = __syncthreads_or(desc);
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        kmin = min(kmin, wave_off[0][w]);
-        kmax = max(kmax, wave_off[1][w]);
-    }
-    // stable LSD radix sort of the window by key - kmin (the sentinel last)
-    int span_bits = 0;
-    if (desc) {
-        const int top = kmax >= kmin ? kmax - kmin + 1 : 1;
-        while ((1 << span_bits) <= top) ++span_bits;
-    }
-    const int passes = (span_bits + kMaxDigitBits - 1) / kMaxDigitBits;
-    const int width = passes > 0 ? (span_bits + passes - 1) / passes : 1;
-    const int nb = 1 << width;
-    const uint64_t lt_mask = (1ull << lane) - 1;
-    int cur = 0;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = p * width;
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wave_off[w][t] = 0;
-        __syncthreads();
-        int32_t kk[PER], ss[PER], rk[PER];
-#pragma unroll
-        for (int r = 0; r < PER; ++r) {
-            const int pos = wave * (PER * 64) + r * 64 + lane;
-            const int32_t key = skey[cur][pos];
-            const int rel = key != sentinel ? key - kmin : kmax - kmin + 1;
-            const int dg = (rel >> shift) & (nb - 1);
-            const uint64_t peers = match_digit(dg, true, width);
-            const int rank = __popcll(peers & lt_mask);
-            // every lane of a digit group reads the counter before its lowest
-            // lane advances it (LDS operations of a wavefront complete in order)
-            const int32_t seen = wave_off[wave][dg];
-            if (rank == 0) wave_off[wave][dg] = seen + __popcll(peers);
-            kk[r] = key;
-            ss[r] = sslot[cur][pos];
-            rk[r] = seen + rank;
-        }
-        __syncthreads();
-        int32_t tot = 0;
-        if (t < nb) {  // exclusive prefix over wavefronts, per digit
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const int32_t c = wave_off[w][t];
-                wave_off[w][t] = tot;
-                tot += c;
-            }
-        }
-        int32_t all;
-        const int32_t st0 = block_exclusive_scan(tot, wsum, all);
-        if (t < nb) start_d[t] = st0;
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < PER; ++r) {
-            const int rel = kk[r] != sentinel ? kk[r] - kmin : kmax - kmin + 1;
-            const int dg = (rel >> shift) & (nb - 1);
-            const int pos = start_d[dg] + wave_off[wave][dg] + rk[r];
-            if (pos >= 0 && pos < W) {  // always true: the counts cover the window
-                skey[cur ^ 1][pos] = kk[r];
-                sslot[cur ^ 1][pos] = ss[r];
-            }
-        }
-        cur ^= 1;
-    }
-    __syncthreads();
-    // lane t takes sorted positions [4t, 4t + 4) (kWindow = 4 kThreads)
-    static_assert(PER == 4, "one chunk of 4 positions per lane");
-    int key[4], sl[4];
-    bool end[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        sl[q] = sslot[cur][4 * t + q] & (W - 1);
-        key[q] = skey[cur][4 * t + q];
-    }
-    const int after = 4 * t + 4 < W ? skey[cur][4 * t + 4] : sentinel;
-    // run ends in this lane (the sentinel run excluded): run r of the window
-    // is the one with r run ends before it
-    int ends = 0, before[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        before[q] = ends;
-        end[q] = key[q] != sentinel && key[q] != (q < 3 ? key[q + 1] : after);
-        ends += end[q];
-    }
-    int32_t nruns;
-    const int32_t base_run = block_exclusive_scan(ends, wsum, nruns);
-    const bool listed = nruns <= kRunCap;
-    // A run is emitted once per wavefront that holds part of it (the lane
-    // scan joins parts inside a wavefront only), so only a run across a
-    // wavefront edge has several emitters: the run holding wavefront w's
-    // first position when the position before has the same key.  Its list
-    // entry is cleared and summed with atomics; every other entry is written
-    // once with plain stores.
     if (lane == 0)
-        edge_run[wave] =
-            wave > 0 && key[0] != sentinel && skey[cur][4 * t - 1] == key[0] ? base_run : -1;
-    __syncthreads();  // the sort buffers are read: the run list may overwrite them
-    if (listed && t < kWaves && edge_run[t] >= 0)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) rw[edge_run[t]][c] = R(0);
+        for (int a = 0; a < 3; ++a) {
+            box[wave][a] = lo[a];
+            box[wave][3 + a] = hi[a];
+        }
     __syncthreads();
-    double ox[4], oy[4], oz[4];
+    int blo[3], bhi[3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        // the window's coordinates again, in sorted order (cache hits;
-        // cheaper than holding the offsets in LDS, which costs occupancy)
-        ox[q] = oy[q] = oz[q] = 0.0;
-        if (key[q] != sentinel) {
-            const int64_t j = base + sl[q];
-            cell_index(parts[j], lx, ihx, ox[q]);
-            cell_index(parts[j + 2 * stride], ly, ihy, oy[q]);
-            cell_index(parts[j + 4 * stride], lz, ihz, oz[q]);
-        } else {
-            key[q] = -1 - (lane * 4 + q);
+    for (int a = 0; a < 3; ++a) {
+        blo[a] = box[0][a];
+        bhi[a] = box[0][3 + a];
+#pragma unroll
+        for (int w = 1; w < kTileWaves; ++w) {
+            blo[a] = min(blo[a], box[w][a]);
+            bhi[a] = max(bhi[a], box[w][3 + a]);
         }
     }
-    if (!listed) {
-        deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, AtomicRuns<R>{rho, gx, gy, gz});
+    if (bhi[0] < 0) return;  // nothing of the window on the grid (block-uniform)
+    // the box of cells lo..hi has (hi - lo + 2) nodes per axis
+    const int nx = bhi[0] - blo[0] + 2, ny = bhi[1] - blo[1] + 2, nz = bhi[2] - blo[2] + 2;
+    const int nodes = nx * ny * nz;
+    if (nodes <= kTileNodes) {  // block-uniform
+        for (int e = t; e < nodes; e += kTileThreads) tile[e] = 0.0;
+        __syncthreads();
+        tile_deposit(tile, blo[0], blo[1], blo[2], nx, ny, lane, w0, ix, iy, iz, ok, ox, oy, oz);
+        __syncthreads();
+        tile_flush<R>(tile, nodes, t, kTileThreads, blo[0], blo[1], blo[2], nx, ny, gx, gy, gz,
+                      rho);
         return;
     }
-    // runs into the list (a run cut by a wavefront edge arrives in parts)
-    static_assert(kWaves == 4, "edge runs of three wavefront edges");
-    const int e1 = edge_run[1], e2 = edge_run[2], e3 = edge_run[3];
-    auto to_list = [&](int k, const CornerSums<R>& a, int q_last) {
-        const int r = base_run + before[q_last];
-        rkey[r] = k;
-        if (r == e1 || r == e2 || r == e3) {
+    // the window's box does not fit: this wavefront's own box in its quarter
+    constexpr int kWaveNodes = kTileNodes / kTileWaves;
+    double* wtile = tile + wave * kWaveNodes;
+    const int wnx = hi[0] - lo[0] + 2, wny = hi[1] - lo[1] + 2, wnz = hi[2] - lo[2] + 2;
+    const int wn = wnx * wny * wnz;
+    if (hi[0] >= 0 && wn <= kWaveNodes) {  // wavefront-uniform; no block barrier below
+        for (int e = lane; e < wn; e += 64) wtile[e] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        tile_deposit(wtile, lo[0], lo[1], lo[2], wnx, wny, lane, w0, ix, iy, iz, ok, ox, oy, oz);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        tile_flush<R>(wtile, wn, lane, 64, lo[0], lo[1], lo[2], wnx, wny, gx, gy, gz, rho);
+        return;
+    }
+    // scattered particles (the sparse tails, or a storage out of cell order):
+    // the quarter becomes a direct-mapped cache of kCacheSlots nodes (key =
+    // node index, claimed by compare-and-swap) summed in FP64; a corner whose
+    // slot holds another node goes to rho directly
+    constexpr int kCacheSlots = kWaveNodes / 2;
+    double* cval = wtile;
+    int* ckey = reinterpret_cast<int*>(wtile + kCacheSlots);
+    for (int e = lane; e < kCacheSlots; e += 64) {
+        cval[e] = 0.0;
+        ckey[e] = -1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int c = 0; c < 8; ++c) atomicAdd(&rw[r][c], R(a.total(c)));
-        } else {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) rw[r][c] = R(a.total(c));
-        }
-    };
-    deposit_runs<R, 4>(key, ox, oy, oz, dom->w0, to_list);
-    __syncthreads();
-    // one addition per node row and cell: corner (0, cy, cz) of cell k also
-    // takes corner (1, cy, cz) of cell k - 1 when that cell is in the list
-    // (same row: k - 1 of a cell with ix = 0 is no cell of the grid)
-    for (int r = t; r < nruns; r += kThreads) {
-        const int k = rkey[r];
-        const bool left = r > 0 && rkey[r - 1] == k - 1;
-        const bool right = r + 1 < nruns && rkey[r + 1] == k + 1;
-        const int ix = k % gx, rr = k / gx;
-        const int iy = rr % gy, iz = rr / gy;
+    for (int q = 0; q < kTilePer; ++q) {
+        if (!ok[q]) continue;
+        double w[8];
+        tile_weights(w0, ox[q], oy[q], oz[q], w);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            if ((c & 1) && right) continue;  // added with cell k + 1
-            R v = rw[r][c];
-            if (!(c & 1) && left) v += rw[r - 1][c + 1];
-            const int cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
-            if (cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1)
-                unsafeAtomicAdd(rho + (int64_t(cz) * gy + cy) * gx + cx, v);
+            const int cx = ix[q] + (c & 1), cy = iy[q] + ((c >> 1) & 1), cz = iz[q] + (c >> 2);
+            if (!(cx > 0 && cx < gx - 1 && cy > 0 && cy < gy - 1 && cz > 0 && cz < gz - 1))
+                continue;
+            const int node = (cz * gy + cy) * gx + cx;
+            const int slot = int((uint32_t(node) * 2654435761u) >> 23) & (kCacheSlots - 1);
+            const int held = atomicCAS(&ckey[slot], -1, node);
+            if (held == -1 || held == node)
+                unsafeAtomicAdd(&cval[slot], w[c]);
+            else
+                unsafeAtomicAdd(rho + node, R(w[c]));
         }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int e = lane; e < kCacheSlots; e += 64) {
+        const int node = ckey[e];
+        if (node >= 0) unsafeAtomicAdd(rho + node, R(cval[e]));
     }
 }
 
@@ -1697,9 +1702,9 @@ static void launch_deposit_local(const double* parts, int64_t stride, const uint
                                  int64_t n, int gx, int gy, int gz, DomainRec* dom, R* rho,
                                  hipStream_t st)
 {
-    const int64_t nb = (n + kWindow - 1) / kWindow;
+    const int64_t nb = (n + kTileWindow - 1) / kTileWindow;
     if (nb < 1) return;
-    hipLaunchKernelGGL(k_deposit_local<R>, dim3(unsigned(nb)), dim3(kThreads), 0, st, parts,
+    hipLaunchKernelGGL(k_deposit_tile<R>, dim3(unsigned(nb)), dim3(kTileThreads), 0, st, parts,
                        stride, masks, n, gx, gy, gz, dom, rho);
 }
 

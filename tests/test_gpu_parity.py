@@ -237,12 +237,12 @@ def test_deposit_windows_out_of_cell_order(gpu, precision):
 
 @pytest.mark.parametrize("precision", [sc.FP64, sc.FP32_GRID])
 def test_deposit_crowded_cells(gpu, precision):
-    """rho when a few cells hold thousands of particles, so that one cell's
-    run fills whole wavefronts and windows and the lane scan carries partial
-    sums across its 16-lane rows, its wavefronts and the window edges: after a
-    reorder (windows in cell order), through the sort permutation (never
-    reordered), and with the storage then shuffled (every window out of
-    order, sorted in LDS)."""
+    """rho when a few cells hold thousands of particles (one cell fills
+    whole wavefronts and windows): after a reorder (compact windows, LDS node
+    tiles), through the sort permutation (never reordered: lane runs and the
+    segmented scan carry partial sums across rows, wavefronts and blocks),
+    and with the storage then shuffled (every window scattered: wavefront
+    node caches, crowded nodes summed in FP64 in LDS)."""
     shape = (16, 16, 32)
     fixed = ((0.0, 0.0, 0.0), (8e-3, 8e-3, 0.1))
     left = np.array([-4e-3, -4e-3, -0.05])
@@ -260,11 +260,10 @@ def test_deposit_crowded_cells(gpu, precision):
     masks = (rng.random(n) > 0.02).astype(np.uint8)
     dt = 0.1 / (BETA * sc.C_LIGHT)
     print(f"case crowded cells precision={precision}", flush=True)
-    # FP32: a crowded cell's total arrives as one once-rounded partial sum
-    # per wavefront (9000 particles: ~35 FP32 atomic adds into each of its
-    # nodes, in an order that varies from run to run; measured 5e-8 to 7e-8
-    # after a reorder, 1.8e-7 to 2.3e-7 through the permutation or shuffled).
-    # A dropped or doubled partial sum would be an error above 1e-3.
+    # FP32: a crowded cell's total arrives as once-rounded FP32 partial sums
+    # (per window or wavefront, FP64 inside it; per run of the permutation
+    # path), added in an order that varies from run to run.  A dropped or
+    # doubled partial sum would be an error above 1e-3.
     tol = 1e-13 if precision == sc.FP64 else 1e-6
     rc, out = oracle_run(oracle_kw(shape, fixed=fixed), parts, masks, 5e10, dt, want=("rho",))
     assert rc == 0
