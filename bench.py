@@ -279,6 +279,18 @@ def timed_sc_run(args, sc, S, config, steps, warmup):
         "roofline": roofline,
     }
     op.set_timing(False)
+    # the same number of steps again with the maps after each kick fused into
+    # the kick kernel (one particle pass for kick + maps, as the served form
+    # runs them): wall time of the tracking loop
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop.run(steps, fuse=True)
+    torch.cuda.synchronize()
+    wall_f = time.perf_counter() - t0
+    rec["config"]["tracking_ms_per_step_fused_maps"] = round(1e3 * wall_f / steps, 4)
+    rec["config"]["tracking_particle_steps_per_s_fused_maps"] = n * steps / wall_f
+    print(f"# tracking with the maps fused into the kick: {1e3 * wall_f / steps:.4f} ms/step -> "
+          f"{n * steps / wall_f:.4e} /s", file=sys.stderr, flush=True)
     del op
     return rec, bunch, opts, prop
 
