@@ -29,6 +29,7 @@
  *                     space_charge_3d_open_hockney.h:22-37) for parity tests.
  *   sc3d_populate_gaussian <- populate_6d, src/synergia/bunch/populate.cc:128-145
  *                     (own counter-based generator, not GSL ranlxd2)
+ *   sc3d_adjust_moments    <- adjust_moments, populate.cc:30-61
  */
 #ifndef SC3D_CAPI_H
 #define SC3D_CAPI_H
@@ -231,6 +232,18 @@ int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n,
                            uint8_t* masks, const double mean[6],
                            const double chol[21], uint64_t seed, int64_t id0,
                            void* stream);
+
+/* adjust_moments (src/synergia/bunch/populate.cc:30-61, the last step of
+ * populate_6d :128-224): maps the first n_active rows so that the bunch's
+ * mean and second moments (over its active particles, divided by total_num)
+ * become exactly `means` and `cov` (6x6 row-major):  (p - mean) A^T + means
+ * with A = G H^-1, G and H the lower Cholesky factors of cov and of the
+ * bunch's second moments (adjust_moments_host, populate_host.cc:8-44).
+ * Errors: the reference's "adjust_moments: covariance matrix must be
+ * symmetric" (SC3D_ERR_ARG), a non-positive-definite matrix (SC3D_ERR_ARG).
+ * Synchronises `stream` once (the moments go to the host for A). */
+int sc3d_adjust_moments(const sc3d_bunch* bunch, const double means[6], const double cov[36],
+                        void* stream);
 
 /* sc3d_moments without a context and without allocating: no reference
  * counterpart; for callers that cannot call hipMalloc (code under stream

@@ -133,6 +133,9 @@ def lib():
         L.orc_det_sincos2pi.argtypes = [C.c_double, dp, dp]
         L.orc_populate_gaussian.argtypes = [dp, C.c_int64, C.c_int64, C.POINTER(C.c_uint8), dp, dp,
                                             C.c_uint64, C.c_int64]
+        L.orc_adjust_matrix.argtypes = [dp, dp, dp]
+        L.orc_adjust_moments.argtypes = [dp, C.c_int64, C.c_int64, C.POINTER(C.c_uint8),
+                                         C.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -305,6 +308,28 @@ def moments(parts, masks, total_num=None):
     lib().orc_moments(_dp(parts), parts.shape[1], n, masks.ctypes.data_as(C.POINTER(C.c_uint8)),
                       float(total_num if total_num is not None else n), _dp(mean), _dp(mom2))
     return mean, mom2.reshape(6, 6)
+
+
+def adjust_matrix(cov, mom2):
+    """(rc, A): A = G H^-1, G and H the lower Cholesky factors of cov and
+    mom2 (adjust_moments_host, populate_host.cc:8-44); rc -1 not symmetric,
+    -2 not positive definite."""
+    A = np.zeros(36)
+    rc = lib().orc_adjust_matrix(_dp(np.ascontiguousarray(cov, np.float64).ravel()),
+                                 _dp(np.ascontiguousarray(mom2, np.float64).ravel()), _dp(A))
+    return rc, A.reshape(6, 6)
+
+
+def adjust_moments(parts, masks, means, cov, total_num=None):
+    """adjust_moments (populate.cc:30-61) in place on parts (7, capacity):
+    the first len(masks) rows are mapped so that the bunch's mean and second
+    moments become means and cov.  Returns rc (0, -1, -2)."""
+    n = masks.shape[0]
+    return lib().orc_adjust_moments(_dp(parts), parts.shape[1], n,
+                                     masks.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                     float(total_num if total_num is not None else n),
+                                     _dp(np.ascontiguousarray(means, np.float64)),
+                                     _dp(np.ascontiguousarray(cov, np.float64).ravel()))
 
 
 def philox4x32_10(ctr, key):

@@ -16,9 +16,14 @@
 //                           negligible extent; hand-computed constants.
 //   orc_cell_index/keys  -- integer closed forms on hand-built coordinates.
 //   orc_stable_sort      -- numpy.argsort(kind="stable").
-//   orc_fft1d/3d         -- numpy.fft golden vectors, roundtrip, Parseval.
+//   orc_fft1d/3d         -- numpy.fft golden vectors (powers of two and
+//                           lengths 12, 30, 74; 3-D 12x10x6, 10x6x14),
+//                           roundtrip, Parseval.
 //   orc_green_fn2        -- independent numpy evaluation of the closed form,
-//                           mirror symmetry, far field -> hz/r.
+//                           mirror symmetry, far field -> hz/r; at the shapes
+//                           where the float-reciprocal index decomposition
+//                           misplaces points, the closed form at each
+//                           misplaced writer's coordinates (or 0).
 //   orc_deposit          -- single-particle closed form, charge
 //                           conservation, brute-force O(8N) numpy deposit.
 //   convolution (apply)  -- brute-force direct sum on a tiny grid.
@@ -31,6 +36,10 @@
 //   orc_philox4x32_10    -- Random123 known-answer vectors; numpy restatement.
 //   orc_det_log/sincos2pi-- 200-bit arithmetic (mpmath), within 1 / 2 ulp.
 //   orc_populate_gaussian-- the pieces above; sample moments vs L L^T.
+//   orc_adjust_matrix    -- numpy cholesky / inverse (1e-14).
+//   orc_adjust_moments   -- the adjusted bunch's mean and second moments
+//                           equal the requested ones (closed form, 1e-13);
+//                           reference's symmetric-check messages.
 #include "sc3d_oracle.h"
 
 #include <algorithm>
@@ -994,6 +1003,93 @@ extern "C" void orc_moments(const double* p, int64_t st, int64_t n, const uint8_
             mom2[j * 6 + k] = mom2[k * 6 + j] = s2[q] / total_num;
             ++q;
         }
+}
+
+// ---------------------------------------------------------------------------
+// adjust_moments (populate.cc:30-61, adjust_moments_host in
+// populate_host.cc:8-44): the particles' mean and second moments are made
+// exactly `means` and `cov`:  A = G H^-1 with G, H the lower Cholesky
+// factors of cov and of the bunch's second moments (Eigen LLT there), then
+// every row p of the first n slots becomes (p - bunch mean) A^T + means.
+// is_symmetric66 (populate.cc:12-27) with floating_point_equal, tolerance
+// 1e-14 (utils/floating_point.h:9-18).  Cholesky-Banachiewicz row by row,
+// sums in ascending k; H^-1 by forward substitution; (p - mean) A^T summed
+// in ascending k.  Returns 0, -1 (not symmetric), -2 (not positive
+// definite).
+// ---------------------------------------------------------------------------
+namespace {
+bool fp_equal(double a, double b, double tol)
+{
+    if (std::abs(a) < tol) return std::abs(a - b) < tol;
+    return std::abs((a - b) / a) < tol;
+}
+
+bool cholesky6(const double* C, double* L)
+{
+    for (int i = 0; i < 36; ++i) L[i] = 0.0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = C[i * 6 + j];
+            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k];
+            if (i == j) {
+                if (!(s > 0.0)) return false;
+                L[i * 6 + i] = std::sqrt(s);
+            } else {
+                L[i * 6 + j] = s / L[j * 6 + j];
+            }
+        }
+    return true;
+}
+
+void lower_inverse6(const double* L, double* M)
+{
+    for (int i = 0; i < 36; ++i) M[i] = 0.0;
+    for (int c = 0; c < 6; ++c) {
+        M[c * 6 + c] = 1.0 / L[c * 6 + c];
+        for (int i = c + 1; i < 6; ++i) {
+            double s = 0.0;
+            for (int k = c; k < i; ++k) s += L[i * 6 + k] * M[k * 6 + c];
+            M[i * 6 + c] = -s / L[i * 6 + i];
+        }
+    }
+}
+}  // namespace
+
+extern "C" int orc_adjust_matrix(const double cov[36], const double mom2[36], double A[36])
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = i + 1; j < 6; ++j)
+            if (!fp_equal(cov[i * 6 + j], cov[j * 6 + i], 1.0e-14)) return -1;
+    double G[36], H[36], Hi[36];
+    if (!cholesky6(cov, G) || !cholesky6(mom2, H)) return -2;
+    lower_inverse6(H, Hi);
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 6; ++k) s += G[i * 6 + k] * Hi[k * 6 + j];
+            A[i * 6 + j] = s;
+        }
+    return 0;
+}
+
+extern "C" int orc_adjust_moments(double* p, int64_t st, int64_t n, const uint8_t* masks,
+                                  double total_num, const double means[6],
+                                  const double cov[36])
+{
+    double mean[6], mom2[36], A[36];
+    orc_moments(p, st, n, masks, total_num, mean, mom2);
+    if (int rc = orc_adjust_matrix(cov, mom2, A)) return rc;
+    for (int64_t i = 0; i < n; ++i) {
+        double d[6], o[6];
+        for (int k = 0; k < 6; ++k) d[k] = p[i + k * st] - mean[k];
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 6; ++k) s += d[k] * A[j * 6 + k];
+            o[j] = s + means[j];
+        }
+        for (int j = 0; j < 6; ++j) p[i + j * st] = o[j];
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -130,6 +130,9 @@ void orc_apply_maps(orc_bunch* b, const orc_slice* s, int nslices);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_det_log(double u);
 void orc_det_sincos2pi(double u, double* sn, double* cs);
+int orc_adjust_matrix(const double cov[36], const double mom2[36], double A[36]);
+int orc_adjust_moments(double* parts, int64_t stride, int64_t n, const uint8_t* masks,
+                       double total_num, const double means[6], const double cov[36]);
 void orc_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                            const double mean[6], const double chol[21], uint64_t seed,
                            int64_t id0);

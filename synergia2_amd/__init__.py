@@ -125,7 +125,7 @@ EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
            "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
            "sc3d_moments", "sc3d_tail_fraction", "sc3d_green_patches",
-           "sc3d_moments_noalloc",
+           "sc3d_moments_noalloc", "sc3d_adjust_moments",
            # served form (include/sc3d_serve.h)
            "sc3d_server_options_default", "sc3d_server_create", "sc3d_server_submit",
            "sc3d_server_iterate", "sc3d_server_job", "sc3d_server_job_particles",
@@ -173,6 +173,9 @@ def load(path=None):
     L.sc3d_moments_noalloc.argtypes = [C.POINTER(BunchDesc), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.c_void_p]
     L.sc3d_moments_noalloc.restype = C.c_int
+    L.sc3d_adjust_moments.argtypes = [C.POINTER(BunchDesc), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.c_void_p]
+    L.sc3d_adjust_moments.restype = C.c_int
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
                  "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
                  "sc3d_get_timing", "sc3d_set_post_kick_maps", "sc3d_moments",
@@ -268,6 +271,23 @@ class Bunch:
 
     def host(self):
         return self.parts[:, :self.n].cpu().numpy().copy(), self.masks[:self.n].cpu().numpy().copy()
+
+    def adjust_moments(self, means, cov, stream=None):
+        """adjust_moments (populate.cc:30-61): make the bunch's mean and second
+        moments exactly means (6) and cov (6x6)."""
+        means = np.ascontiguousarray(means, np.float64)
+        cov = np.ascontiguousarray(cov, np.float64).reshape(36)
+        d = self.desc()
+        _check(load().sc3d_adjust_moments(C.byref(d), means.ctypes.data_as(C.POINTER(C.c_double)),
+                                          cov.ctypes.data_as(C.POINTER(C.c_double)),
+                                          _stream_ptr(stream)))
+
+    def populate_6d(self, means, cov, seed, stream=None):
+        """populate_6d (populate.cc:128-145): unit 6D Gaussians (the
+        counter-based generator in place of GSL's), then adjust_moments to
+        means and cov."""
+        self.populate_gaussian(np.zeros(6), diag_chol([1.0] * 6), seed, stream=stream)
+        self.adjust_moments(means, cov, stream)
 
     def populate_gaussian(self, mean, chol, seed, id0=0, stream=None):
         mean = np.ascontiguousarray(mean, np.float64)

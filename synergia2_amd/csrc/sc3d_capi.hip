@@ -682,6 +682,91 @@ int sc3d_tail_fraction(sc3d_ctx* c, const sc3d_bunch* b, const double center[6],
     return SC3D_OK;
 }
 
+int sc3d_adjust_moments(const sc3d_bunch* b, const double means[6], const double cov[36],
+                        void* stream)
+{
+    if (!b || !means || !cov) return fail(SC3D_ERR_ARG, "adjust_moments: null argument");
+    const int64_t n = b->n_active;
+    if (n < 1 || b->stride < n || !b->parts || !b->masks || !(b->total_num > 0.0))
+        return fail(SC3D_ERR_ARG, "adjust_moments: bad bunch");
+    // is_symmetric66 (populate.cc:12-27): floating_point_equal, tolerance 1e-14
+    auto fp_equal = [](double a, double c) {
+        return std::abs(a) < 1.0e-14 ? std::abs(a - c) < 1.0e-14
+                                     : std::abs((a - c) / a) < 1.0e-14;
+    };
+    for (int i = 0; i < 6; ++i)
+        for (int j = i + 1; j < 6; ++j)
+            if (!fp_equal(cov[i * 6 + j], cov[j * 6 + i]))
+                return fail(SC3D_ERR_ARG, "adjust_moments: covariance matrix must be symmetric");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the bunch's mean and second moments (sc3d_moments' kernels); the
+    // partials live only for this call
+    const int64_t parts_n = ((n + 4095) / 4096) * 21 + 64;
+    double* P = nullptr;
+    HIP_TRY(hipMalloc(&P, size_t(parts_n + 27) * sizeof(double)));
+    launch_moments(b->parts, b->stride, b->masks, n, b->total_num, P, P + parts_n, st);
+    double h[27];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, P + parts_n, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(P);
+    if (e != hipSuccess)
+        return fail(SC3D_ERR_HIP, std::string("adjust_moments: ") + hipGetErrorString(e));
+    double mom2[36];
+    int q = 6;
+    for (int j = 0; j < 6; ++j)
+        for (int k = 0; k <= j; ++k) {
+            mom2[j * 6 + k] = mom2[k * 6 + j] = h[q];
+            ++q;
+        }
+    // A = G H^-1 (Eigen LLT factors there): Cholesky-Banachiewicz, sums in
+    // ascending k; H^-1 by forward substitution
+    auto chol = [](const double* C, double* L) {
+        for (int i = 0; i < 36; ++i) L[i] = 0.0;
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double s = C[i * 6 + j];
+                for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k];
+                if (i == j) {
+                    if (!(s > 0.0)) return false;
+                    L[i * 6 + i] = std::sqrt(s);
+                } else {
+                    L[i * 6 + j] = s / L[j * 6 + j];
+                }
+            }
+        return true;
+    };
+    double G[36], H[36], Hi[36];
+    if (!chol(cov, G))
+        return fail(SC3D_ERR_ARG, "adjust_moments: covariance matrix is not positive definite");
+    if (!chol(mom2, H))
+        return fail(SC3D_ERR_ARG,
+                    "adjust_moments: the bunch's second moments are not positive definite");
+    for (int i = 0; i < 36; ++i) Hi[i] = 0.0;
+    for (int c = 0; c < 6; ++c) {
+        Hi[c * 6 + c] = 1.0 / H[c * 6 + c];
+        for (int i = c + 1; i < 6; ++i) {
+            double s = 0.0;
+            for (int k = c; k < i; ++k) s += H[i * 6 + k] * Hi[k * 6 + c];
+            Hi[i * 6 + c] = -s / H[i * 6 + i];
+        }
+    }
+    AdjustArgs a{};
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 6; ++k) s += G[i * 6 + k] * Hi[k * 6 + j];
+            a.A[i * 6 + j] = s;
+        }
+    for (int j = 0; j < 6; ++j) {
+        a.mean[j] = h[j];
+        a.means[j] = means[j];
+    }
+    launch_adjust(b->parts, b->stride, n, a, st);
+    HIP_TRY(hipGetLastError());
+    return SC3D_OK;
+}
+
 int sc3d_populate_gaussian(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                            const double mean[6], const double chol[21], uint64_t seed,
                            int64_t id0, void* stream)

@@ -94,5 +94,13 @@ void launch_tail_counts(const double* parts, int64_t stride, const uint8_t* mask
                         const TailArgs& a, unsigned long long* counts, hipStream_t st);
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                      const PopulateArgs& a, hipStream_t st);
+// adjust_moments' particle map (populate_host.cc:30-43): every row i < n
+// becomes (p - mean) A^T + means, sums in ascending k.
+struct AdjustArgs {
+    double A[36];
+    double mean[6];   // the bunch's mean
+    double means[6];  // the requested mean
+};
+void launch_adjust(double* parts, int64_t stride, int64_t n, const AdjustArgs& a, hipStream_t st);
 
 }  // namespace sc3d

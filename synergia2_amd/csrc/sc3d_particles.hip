@@ -1589,6 +1589,33 @@ void launch_tail_counts(const double* parts, int64_t stride, const uint8_t* mask
 }
 
 
+__global__ __launch_bounds__(kThreads) void k_adjust(double* __restrict__ parts, int64_t stride,
+                                                     int64_t n, AdjustArgs a)
+{
+    for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kThreads) {
+        double d[6], o[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) d[k] = parts[i + k * stride] - a.mean[k];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += d[k] * a.A[j * 6 + k];
+            o[j] = s + a.means[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) parts[i + j * stride] = o[j];
+    }
+}
+
+void launch_adjust(double* parts, int64_t stride, int64_t n, const AdjustArgs& a, hipStream_t st)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_adjust, dim3(grid_for(n)), dim3(kThreads), 0, st, parts, stride, n,
+                           a);
+}
+
 void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                    double* P, const DomainFinishArgs& a, DomainRec* dom, const Sched& sc,
                    hipStream_t st)

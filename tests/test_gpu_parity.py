@@ -760,3 +760,39 @@ def test_any_shape_vs_oracle(gpu, shape):
             ((sc.DBG_EX, "ex"), (sc.DBG_EY, "ey"), (sc.DBG_EZ, "ez"))]
     print(f"  precision form field rel-L2 {errs}", flush=True)
     assert max(errs) <= 2e-6
+
+
+def test_adjust_moments_bit_exact_vs_oracle(gpu):
+    """adjust_moments (populate.cc:30-61) on the device equals the oracle's
+    bit for bit (same fixed-tree moments, same Cholesky / inverse order, same
+    per-particle sums), and the adjusted bunch has the requested mean and
+    second moments; populate_6d = unit Gaussians + adjust_moments; the
+    reference's symmetric-check error."""
+    n = 200003
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((6, 6)) * np.array([1e-3, 2e-4, 3e-3, 4e-4, 1e-2, 1e-4])[:, None]
+    cov = A @ A.T
+    means = np.array([1e-4, -2e-5, 3e-4, 1e-6, -1e-3, 2e-6])
+    b = sc.Bunch(n, 1e10, gamma=GAMMA, capacity=n + 11)
+    b.populate_gaussian(np.zeros(6), sc.diag_chol([1.0] * 6), seed=9)
+    b.masks[::13] = 0
+    p0, m0 = b.host()
+    b.adjust_moments(means, cov)
+    pg, _ = b.host()
+    po = np.ascontiguousarray(p0.copy())
+    assert O.adjust_moments(po, np.ascontiguousarray(m0), means, cov, total_num=n) == 0
+    assert np.array_equal(pg.view(np.int64), po.view(np.int64))
+    op = sc.SpaceCharge3dOpenHockney(sc.default_options(32, 32, 64))
+    mean, mom2 = op.moments(b)
+    assert np.allclose(mean, means, rtol=1e-12, atol=1e-16)
+    assert np.allclose(mom2, cov, rtol=1e-11, atol=1e-13 * np.abs(cov).max())
+    b2 = sc.Bunch(n, 1e10, gamma=GAMMA)
+    b2.populate_6d(means, cov, seed=9)
+    mean2, mom22 = op.moments(b2)
+    assert np.allclose(mean2, means, rtol=1e-12, atol=1e-16)
+    assert np.allclose(mom22, cov, rtol=1e-11, atol=1e-13 * np.abs(cov).max())
+    bad = cov.copy()
+    bad[0, 1] *= 1 + 1e-10
+    with pytest.raises(sc.Sc3dError) as ei:
+        b2.adjust_moments(means, bad)
+    assert "adjust_moments: covariance matrix must be symmetric" in str(ei.value)
