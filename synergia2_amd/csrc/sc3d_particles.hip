@@ -1019,11 +1019,16 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // consecutive per thread 81-99; tiles of 2048 / 3072 nodes 93-105 / 83-93;
 // 512-thread blocks (6144 / 8192 nodes) 61-102 / 60-75; without the
 // per-wavefront fallback and the block-order rotation 68-78; the corner sums
-// picked by an indexed load (through scratch) 107-134.
+// picked by an indexed load (through scratch) 107-134.  In the library
+// (deposit phase per step, 4e6 FP64 / 1.6e7 FP32): 4080 nodes (32 KiB with
+// the box, 5 blocks per CU) 56.5 / 224.7 against 55.7 / 215.5; the same at 5
+// waves per SIMD (96 VGPRs, 32 bytes spilled) 60.0 / 228.8; 3060 nodes at 6
+// waves 93.3 / 351.9; the last eighth rotated first 55.8 / 215.7.
 constexpr int kTileThreads = 256;
 constexpr int kTilePer = 4;                             // consecutive slots per thread
 constexpr int kTileWindow = kTileThreads * kTilePer;    // slots per block
 constexpr int kTileNodes = 4096;                        // FP64 nodes of the LDS tile (32 KiB)
+constexpr int kTileTailDiv = 16;                        // 1/16 of the windows rotated first
 constexpr int kTileWaves = kTileThreads / 64;
 
 __device__ __forceinline__ int wave_min(int v)
@@ -1144,7 +1149,7 @@ __global__ __launch_bounds__(kTileThreads) void k_deposit_tile(
     // precision form: unit weights (w0 and the unit-peak scale return at the kick)
     const double w0 = sizeof(R) == sizeof(double) ? dom->w0 : 1.0;
     // the last sixteenth of the windows (one tail of the bunch) first
-    const int nb = int(gridDim.x), tail = nb / 16;
+    const int nb = int(gridDim.x), tail = nb / kTileTailDiv;
     const int blk = (int(blockIdx.x) + nb - tail) % nb;
     const int64_t base = int64_t(blk) * kTileWindow + int64_t(t) * kTilePer;
     int ix[kTilePer], iy[kTilePer], iz[kTilePer];

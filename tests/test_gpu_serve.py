@@ -53,11 +53,16 @@ def close(a, b, tol):
     assert max(errs) <= tol, errs
 
 
-@pytest.mark.parametrize("precision,tol", [(sc.FP64, 1e-10), (sc.FP32_GRID, 1e-6)])
-def test_served_job_matches_operator_api(gpu, precision, tol):
+@pytest.mark.parametrize("precision,tol,shape", [(sc.FP64, 1e-10, (32, 32, 64)),
+                                                 (sc.FP32_GRID, 1e-6, (32, 32, 64)),
+                                                 (sc.FP64, 1e-10, (24, 24, 48)),
+                                                 (sc.FP32_GRID, 1e-6, (48, 48, 96))])
+def test_served_job_matches_operator_api(gpu, precision, tol, shape):
     """One job in the server == the bunch tracked by the Propagator over the
-    operator API with the same options (same sort_period)."""
-    job = SV.Job(30000, (32, 32, 64), 23, 1.5e-3, 1.0e-3, 0.012, seed=11)
+    operator API with the same options (same sort_period); also on grids
+    of other lengths (generic transform passes; at 48x48x96 with the
+    reference's g2 correction spectrum)."""
+    job = SV.Job(30000, shape, 23, 1.5e-3, 1.0e-3, 0.012, seed=11)
     srv = SV.Server(SV.options(precision=precision, keep_particles=True))
     jid = run_all(srv, [job])[0]
     res = srv.result(jid)
