@@ -71,13 +71,6 @@ template <int M> struct XLines {
     static constexpr int value = M >= 128 ? Lines<M>::value : HalfLines<M>::value;
 };
 
-// Column passes: enough lines per block that each strided load or store
-// moves a 128-byte run (16 FP32 or 8 FP64 complex values).
-template <int L, typename T2> struct ColLines {
-    static constexpr int runs = 128 / int(sizeof(T2));
-    static constexpr int value = Lines<L>::value > runs ? Lines<L>::value : runs;
-};
-
 // Lines per block, from a preferred count, halved (down to 2) while the
 // block's LDS (K lines per transform line plus the twiddle table) exceeds
 // the cap: the passes are latency-bound at low occupancy, and gfx950 has
@@ -101,9 +94,6 @@ template <int L, int K, typename T2, int PREF> struct CappedLines {
     static constexpr int fit(int nl) { return (nl > 2 && bytes(nl) > cap) ? fit(nl / 2) : nl; }
     static constexpr int value = fit(PREF);
 };
-// column passes holding K lines per transform line
-template <int L, int K, typename T2>
-using FieldLines = CappedLines<L, K, T2, ColLines<L, T2>::value>;
 // row passes holding K lines per row (rows are contiguous: no run width)
 template <int M, int K, typename T2>
 using RowLines = CappedLines<M, K, T2, HalfLines<M>::value>;
@@ -111,10 +101,66 @@ using RowLines = CappedLines<M, K, T2, HalfLines<M>::value>;
 template <int L, typename T2>
 using GreenLines = CappedLines<L, 1, T2, HalfLines<L>::value>;
 
+// a * w^m, w = e^(-2 pi i / 16) (inverse: e^(+2 pi i / 16)), written with
+// the cosine c and sine s of 2 pi m / 16 (fused multiply-adds: the build
+// contracts nothing by itself)
+template <bool INV, typename T2, typename F>
+__device__ __forceinline__ T2 rot16(T2 a, F c, F s)
+{
+    T2 o;
+    if (!INV) { o.x = fma(a.x, c, a.y * s); o.y = fma(a.y, c, -(a.x * s)); }
+    else      { o.x = fma(a.x, c, -(a.y * s)); o.y = fma(a.y, c, a.x * s); }
+    return o;
+}
+
+// a * w for a twiddle w (INV: a * conj(w)), fused multiply-adds
+template <bool INV, typename T2>
+__device__ __forceinline__ T2 twmul(T2 a, T2 w)
+{
+    T2 o;
+    if (!INV) { o.x = fma(a.x, w.x, -(a.y * w.y)); o.y = fma(a.x, w.y, a.y * w.x); }
+    else      { o.x = fma(a.x, w.x, a.y * w.y);    o.y = fma(a.y, w.x, -(a.x * w.y)); }
+    return o;
+}
+
 template <int R, bool INV, typename T2>
 __device__ __forceinline__ void dft_small(T2* v)
 {
-    if constexpr (R == 8) {
+    if constexpr (R == 16) {
+        // 16 = 4 x 4: inputs x[4 n1 + n2]; a length-4 DFT over n1 for each
+        // n2 gives A[n2][k1], then A[n2][k1] w^(n2 k1), then a length-4 DFT
+        // over n2 for each k1 gives X[k1 + 4 k2]
+        using F = decltype(v[0].x);
+        const F c1 = F(0.92387953251128675613), s1 = F(0.38268343236508977173);
+        const F c2 = F(0.70710678118654752440);
+        T2 a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = v[4 * n1 + n2];
+            dft_small<4, INV>(a[n2]);
+        }
+        a[1][1] = rot16<INV>(a[1][1], c1, s1);   // w^1
+        a[1][2] = rot16<INV>(a[1][2], c2, c2);   // w^2
+        a[1][3] = rot16<INV>(a[1][3], s1, c1);   // w^3
+        a[2][1] = rot16<INV>(a[2][1], c2, c2);   // w^2
+        {                                        // w^4 = -i (inverse: +i)
+            T2 t = a[2][2];
+            if (!INV) { a[2][2].x = t.y;  a[2][2].y = -t.x; }
+            else      { a[2][2].x = -t.y; a[2][2].y = t.x; }
+        }
+        a[2][3] = rot16<INV>(a[2][3], -c2, c2);  // w^6
+        a[3][1] = rot16<INV>(a[3][1], s1, c1);   // w^3
+        a[3][2] = rot16<INV>(a[3][2], -c2, c2);  // w^6
+        a[3][3] = rot16<INV>(a[3][3], -c1, -s1); // w^9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            T2 b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            dft_small<4, INV>(b);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+        }
+    } else if constexpr (R == 8) {
         // y_k = E_(k mod 4) + w8^k O_(k mod 4), E/O = DFT4 of the even/odd
         // inputs, w8 = e^(-i pi/4) (inverse: conjugate), w8^(k+4) = -w8^k
         T2 e[4] = {v[0], v[2], v[4], v[6]};
@@ -377,74 +423,241 @@ __global__ __launch_bounds__(kThreads) void k_x_c2r(const GridJob* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Column passes over S.  Line l -> (plane = l / lpp, c = l % lpp), element p
-// at S[plane*plane_stride + p*pstride + c].  Loads p < nload (rest zero),
-// stores p < nstore.  MODE 0: forward; 1: inverse; 2: forward, multiply by
-// the real Green's spectrum, inverse (the z pass).
+// Register-resident column transforms.  A line of length L is held by
+// TPL = L / E threads with E elements each (E = the line's largest radix);
+// a radix-r pass is E / r butterflies per thread in registers, and only the
+// exchanges between passes go through LDS (one line buffer per line).
+// Thread j of a line owns the butterflies b = j + q * TPL (q < E / r) of a
+// radix-r pass, whose inputs are the elements b + s * L / r; it writes its
+// outputs to the Stockham positions (b / Ns) * Ns * r + b % Ns + s * Ns.  In
+// the last pass (Ns * r = L) those are the input positions again, so a
+// forward transform that ends with radix r and an inverse one that starts
+// with it (the inverse runs the radices in reverse) hold the same elements
+// in the same registers: the z pass multiplies by the Green's spectrum in
+// registers and goes straight on with the inverse transform.
 // ---------------------------------------------------------------------------
+template <int L, int W> struct RegShape {
+    // forward radices, first to last; W = 16: radix-16 passes where L allows
+    // (two passes for 128 and 256), W = 8: radix-8 passes and one smaller
+    static constexpr int np = (W == 16 ? L == 512 : L >= 128) ? 3 : (L <= 16 ? 1 : 2);
+    static constexpr int r0 = (L == 16 || (W == 16 && (L == 128 || L == 256))) ? 16 : 8;
+    static constexpr int r1 = L == 32 ? 4 : ((W == 16 && L == 256) ? 16 : (L >= 64 ? 8 : 1));
+    static constexpr int r2 = L == 512 ? 8 : ((W == 8 && L == 128) ? 2 : ((W == 8 && L == 256) ? 4 : 1));
+    static constexpr int E = r0;
+    static constexpr int TPL = L / E;
+    static constexpr int radix(int p, bool inv)
+    {
+        const int q = inv ? np - 1 - p : p;
+        return q == 0 ? r0 : (q == 1 ? r1 : r2);
+    }
+    static constexpr int stage(int p, bool inv)
+    {
+        int ns = 1;
+        for (int i = 0; i < p; ++i) ns *= radix(i, inv);
+        return ns;
+    }
+    static_assert(L <= 16 || r0 * r1 * r2 == L, "radices multiply to L");
+};
+// The plan of a precision: radix 16 for FP64 lines; radix 8 for FP32 lines,
+// twice the threads per line (measured at 128x128x256 FP32: y' pass 61.3 ->
+// 50.1 us, y pass 22.2 -> 20.9 us; at 64x64x128 FP64 the radix-16 z pass is
+// 16.6 us against 17.6)
+template <int L, typename T2> using RegPlan = RegShape<L, (sizeof(T2) >= 16 ? 16 : 8)>;
+
+template <typename T2, int L, int P, bool INV>
+__device__ __forceinline__ void reg_pass(T2 (&v)[(RegPlan<L, T2>::E)], int j, const T2* __restrict__ tw)
+{
+    using S = RegPlan<L, T2>;
+    constexpr int r = S::radix(P, INV), NS = S::stage(P, INV), Q = S::E / r;
+    constexpr int SH = L / (NS * r);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if constexpr (NS > 1) {
+            const int k = (j + q * S::TPL) & (NS - 1);
+#pragma unroll
+            for (int s = 1; s < r; ++s) v[q * r + s] = twmul<INV>(v[q * r + s], tw[s * k * SH]);
+        }
+        dft_small<r, INV>(v + q * r);
+    }
+}
+
+// Line buffers are L elements apart with no padding; element p of a line
+// sits at p ^ sw, sw = line mod G, G = the elements in 128 bytes (at most
+// L): the lanes of a wavefront that touch the same element of G
+// neighbouring lines then hit G different bank groups.
+template <int L, typename T2> struct RegSwz {
+    static constexpr int G0 = 128 / int(sizeof(T2));
+    static constexpr int G = G0 < L ? G0 : L;
+};
+
+// outputs of pass P -> the line buffer
+template <typename T2, int L, int P, bool INV>
+__device__ __forceinline__ void reg_to_lds(const T2 (&v)[(RegPlan<L, T2>::E)], T2* lb, int j, int sw)
+{
+    using S = RegPlan<L, T2>;
+    constexpr int r = S::radix(P, INV), NS = S::stage(P, INV), Q = S::E / r;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = j + q * S::TPL;
+        const int d = (b / NS) * NS * r + (b & (NS - 1));
+#pragma unroll
+        for (int s = 0; s < r; ++s) lb[(d + s * NS) ^ sw] = v[q * r + s];
+    }
+}
+
+// inputs of pass P <- the line buffer
+template <typename T2, int L, int P, bool INV>
+__device__ __forceinline__ void reg_from_lds(T2 (&v)[(RegPlan<L, T2>::E)], const T2* lb, int j, int sw)
+{
+    using S = RegPlan<L, T2>;
+    constexpr int r = S::radix(P, INV), Q = S::E / r, B = L / r;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int b = j + q * S::TPL;
+#pragma unroll
+        for (int s = 0; s < r; ++s) v[q * r + s] = lb[(b + s * B) ^ sw];
+    }
+}
+
+// One unnormalised length-L transform of the line held in v (the inputs of
+// pass 0 on entry, the outputs of the last pass on return).  lb is the
+// line's LDS buffer and sw its swizzle, tw the length-L twiddle table (the
+// block's LDS copy, filled before the call); busy: the block has used its
+// line buffers before (a barrier then precedes the first write).  Every
+// thread of the block calls it (it holds block barriers).
+template <typename T2, int L, bool INV>
+__device__ __forceinline__ void reg_fft(T2 (&v)[(RegPlan<L, T2>::E)], T2* lb, int j, int sw,
+                                        const T2* __restrict__ tw, bool busy)
+{
+    using S = RegPlan<L, T2>;
+    reg_pass<T2, L, 0, INV>(v, j, tw);
+    if constexpr (S::np > 1) {
+        if (busy) __syncthreads();
+        reg_to_lds<T2, L, 0, INV>(v, lb, j, sw);
+        __syncthreads();
+        reg_from_lds<T2, L, 1, INV>(v, lb, j, sw);
+        reg_pass<T2, L, 1, INV>(v, j, tw);
+    }
+    if constexpr (S::np > 2) {
+        __syncthreads();
+        reg_to_lds<T2, L, 1, INV>(v, lb, j, sw);
+        __syncthreads();
+        reg_from_lds<T2, L, 2, INV>(v, lb, j, sw);
+        reg_pass<T2, L, 2, INV>(v, j, tw);
+    }
+}
+
+// Lines per block of the register-resident column passes: 128-byte runs
+// along the line direction (8 FP64 or 16 FP32 complex values), fewer where
+// the block's LDS would pass the cap, and at least one wavefront of
+// threads.
+#ifndef SC3D_REG_RUN_BYTES
+#define SC3D_REG_RUN_BYTES 128
+#endif
+#ifndef SC3D_REG_LDS_CAP
+#define SC3D_REG_LDS_CAP 40960
+#endif
+template <int L, typename T2> struct RegLines {
+    static constexpr int TPL = RegPlan<L, T2>::TPL;
+    static constexpr size_t bytes(int nl) { return size_t(nl + 1) * L * sizeof(T2); }
+    static constexpr int fit(int nl) { return (nl > 1 && bytes(nl) > SC3D_REG_LDS_CAP) ? fit(nl / 2) : nl; }
+    static constexpr int runs = SC3D_REG_RUN_BYTES / int(sizeof(T2));
+    static constexpr int wave = 64 / TPL > 1 ? 64 / TPL : 1;
+    static constexpr int value = fit(runs > wave ? runs : wave);
+    static constexpr int threads = value * TPL;
+};
+
+// Column passes over S.  Line l -> (plane = l / lpp, c = l % lpp), element p
+// at S[plane*plane_stride + p*pstride + c].  MODE 0: forward, loads the
+// non-zero half p < L/2; 1: inverse, stores the kept half p < L/2; 2 (the z
+// pass): forward, multiply by the real Green's spectrum, inverse, half
+// loaded and half stored.
 template <typename R, int L, int MODE>
-__global__ __launch_bounds__(kThreads) void k_cols(const GridJob* __restrict__ jobs,
-                                                   const typename cplx<R>::type* __restrict__ tw,
-                                                   int nlines, int lpp, int64_t plane_stride,
-                                                   int64_t pstride, int nload, int nstore, int gy,
-                                                   int Hx)
+__global__ __launch_bounds__((RegLines<L, typename cplx<R>::type>::threads)) void k_cols_reg(
+    const GridJob* __restrict__ jobs, const typename cplx<R>::type* __restrict__ tw, int nlines,
+    int lpp, int64_t plane_stride, int64_t pstride, int gy, int Hx)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = FieldLines<L, 1, T2>::value;
-    __shared__ T2 buf[NL * LdsLine<L>::stride];
+    using S = RegPlan<L, T2>;
+    constexpr int NL = RegLines<L, T2>::value;
+    constexpr int E = S::E, TPL = S::TPL;
+    __shared__ T2 buf[NL * L];
     __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
-    T2* __restrict__ S = static_cast<T2*>(J.S);
-    const R* __restrict__ ghat = static_cast<const R*>(J.ghat);
+    T2* __restrict__ S_ = static_cast<T2*>(J.S);
     const DomainRec* dom = J.dom;
     if (dom->error) return;
-    const int l0 = xcd_group() * NL;
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        T2 v;
-        if (p < nload && l < nlines) {
-            int plane = l / lpp, c = l - plane * lpp;
-            v = S[plane * plane_stride + p * pstride + c];
-        } else {
-            v.x = R(0); v.y = R(0);
-        }
-        buf[LdsLine<L>::at(line, p)] = v;
+    const int line = int(threadIdx.x) % NL, j = int(threadIdx.x) / NL;
+    const int l = xcd_group() * NL + line;
+    const bool live = l < nlines;
+    int64_t base = 0;
+    if (live) {
+        const int plane = l / lpp, c = l - plane * lpp;
+        base = plane * plane_stride + c;
     }
-    load_twiddles<T2, L>(twl, tw);
-    __syncthreads();
-    if constexpr (MODE == 1) {
-        lds_fft<T2, L, NL, true>(buf, twl);
-    } else {
-        lds_fft<T2, L, NL, false>(buf, twl);
-    }
+    T2* lb = buf + line * L;
+    const int sw = line & (RegSwz<L, T2>::G - 1);
+    for (int e = threadIdx.x; e < L; e += NL * TPL) twl[e] = tw[e];
+    // MODE 2: the Green's values of the elements this thread multiplies (the
+    // outputs of the last forward pass), fetched with the line's loads
+    R gv[E];
     if constexpr (MODE == 2) {
-        // line l = ky*Hx + kx over the full Dy rows; element p = kz.
-        const int Dy2 = lpp / Hx;  // Dy
-        for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-            int p = e / NL, line = e % NL;
-            int l = l0 + line;
-            if (l >= nlines) continue;
-            int ky = l / Hx, kx = l - ky * Hx;
-            int my = ky <= Dy2 / 2 ? ky : Dy2 - ky;
-            int mz = p <= L / 2 ? p : L - p;
-            R gv = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
-            T2 v = buf[LdsLine<L>::at(line, p)];
-            v.x = v.x * gv;
-            v.y = v.y * gv;
-            buf[LdsLine<L>::at(line, p)] = v;
-        }
-        __syncthreads();
-        lds_fft<T2, L, NL, true>(buf, twl);
+        const R* __restrict__ ghat = static_cast<const R*>(J.ghat);
+        constexpr int rl = S::radix(S::np - 1, false), Bl = L / rl, Ql = E / rl;
+        const int Dy = lpp / Hx;
+        const int ky = l / Hx, kx = l - ky * Hx;
+        const int my = ky <= Dy / 2 ? ky : Dy - ky;
+#pragma unroll
+        for (int q = 0; q < Ql; ++q)
+#pragma unroll
+            for (int s = 0; s < rl; ++s) {
+                const int p = j + q * TPL + s * Bl;
+                const int mz = p <= L / 2 ? p : L - p;
+                gv[q * rl + s] = live ? ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx] : R(0);
+            }
     }
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        if (p < nstore && l < nlines) {
-            int plane = l / lpp, c = l - plane * lpp;
-            S[plane * plane_stride + p * pstride + c] = buf[LdsLine<L>::at(line, p)];
+    // inputs of the first pass: elements b + s * L / r, b = j + q * TPL; the
+    // forward passes start with r = E (one butterfly: elements j + s * TPL,
+    // so p < L/2 is s < E/2)
+    constexpr bool INV0 = MODE == 1;
+    constexpr int rf = S::radix(0, INV0), Bf = L / rf, Qf = E / rf;
+    static_assert(MODE == 1 || Qf == 1, "forward passes start with the widest radix");
+    T2 v[E];
+#pragma unroll
+    for (int q = 0; q < Qf; ++q)
+#pragma unroll
+        for (int s = 0; s < rf; ++s) {
+            const int p = j + q * TPL + s * Bf;
+            const bool in = MODE == 1 || s < rf / 2;
+            if (in && live) {
+                v[q * rf + s] = S_[base + p * pstride];
+            } else {
+                v[q * rf + s].x = R(0);
+                v[q * rf + s].y = R(0);
+            }
         }
+    reg_fft<T2, L, MODE == 1>(v, lb, j, sw, twl, false);
+    if constexpr (MODE == 2) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            v[e].x = v[e].x * gv[e];
+            v[e].y = v[e].y * gv[e];
+        }
+        reg_fft<T2, L, true>(v, lb, j, sw, twl, true);
     }
+    // outputs of the last pass: elements b + s * L / r; the inverse passes
+    // end with r = E (p < L/2 is s < E/2)
+    constexpr bool INVL = MODE != 0;
+    constexpr int rl = S::radix(S::np - 1, INVL), Bl = L / rl, Ql = E / rl;
+    static_assert(MODE == 0 || Ql == 1, "inverse passes end with the widest radix");
+    if (!live) return;
+#pragma unroll
+    for (int q = 0; q < Ql; ++q)
+#pragma unroll
+        for (int s = 0; s < rl; ++s) {
+            const int p = j + q * TPL + s * Bl;
+            if (MODE == 0 || s < rl / 2) S_[base + p * pstride] = v[q * rl + s];
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -484,156 +697,150 @@ __device__ __forceinline__ auto sin_2pi(const T2* __restrict__ tw, int k, int L)
     return (k == 0 || 2 * k == L) ? decltype(tw[0].y)(0) : -tw[k].y;
 }
 
-// The z pass holds one LDS line per column: the Green-multiplied spectrum
-// stays in registers (each thread keeps the elements it multiplied) for the
-// second inverse transform, and the Green values are fetched before the
-// forward transform so that both global loads are in flight together.
-// Lines per block: at most SC3D_ZF_LINES (measured at 128x128x256: 4 lines
-// 287 us, 8 lines 323 us, 16 lines 424 us for the FP32 field passes; the
-// pass is latency-bound and more lines cost occupancy; later, with the
-// current passes, 2 lines 287 us against 266 with 4: 16-byte runs).
-#ifndef SC3D_ZF_LINES
-#define SC3D_ZF_LINES 4
-#endif
-template <int L, typename T2>
-struct ZFieldLines {
-    static constexpr int capped = FieldLines<L, 1, T2>::value;
-    static constexpr int value = capped < SC3D_ZF_LINES ? capped : SC3D_ZF_LINES;
-};
-
+// The z' and y' passes register-resident (the register layout of the
+// column passes above).  z': forward, x Ghat in registers, a copy kept for
+// T * D_z, the two inverse transforms one after the other through the same
+// line buffer.  y': the three inverse transforms likewise.  Every load of a
+// line precedes the barrier of its first exchange, so the in-place stores
+// cannot overtake a load (L <= 16: one thread per line).
 template <typename R, int L>
-__global__ __launch_bounds__(kThreads) void k_z_field(const GridJob* __restrict__ jobs,
-                                                      const typename cplx<R>::type* __restrict__ tw,
-                                                      int nlines, int gz, int gy, int Hx)
+__global__ __launch_bounds__((RegLines<L, typename cplx<R>::type>::threads)) void k_z_field_reg(
+    const GridJob* __restrict__ jobs, const typename cplx<R>::type* __restrict__ tw, int nlines,
+    int gy, int Hx)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = ZFieldLines<L, T2>::value;
-    constexpr int PER = (NL * L + kThreads - 1) / kThreads;
-    __shared__ T2 buf[NL * LdsLine<L>::stride];
+    using S = RegPlan<L, T2>;
+    constexpr int NL = RegLines<L, T2>::value;
+    constexpr int E = S::E, TPL = S::TPL;
+    __shared__ T2 buf[NL * L];
     __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
-    T2* __restrict__ S = static_cast<T2*>(J.S);
+    T2* __restrict__ S1 = static_cast<T2*>(J.S);
     T2* __restrict__ S2 = static_cast<T2*>(J.S2);
     const R* __restrict__ ghat = static_cast<const R*>(J.ghat);
     const DomainRec* dom = J.dom;
     if (dom->error) return;
     const R ihz = R(dom->inv_h[2]);
-    const int l0 = xcd_group() * NL;
+    const int line = int(threadIdx.x) % NL, j = int(threadIdx.x) / NL;
+    const int l = xcd_group() * NL + line;
+    const bool live = l < nlines;
+    T2* lb = buf + line * L;
+    const int sw = line & (RegSwz<L, T2>::G - 1);
+    for (int e = threadIdx.x; e < L; e += NL * TPL) twl[e] = tw[e];
+    // Green's values of the outputs of the last forward pass
+    constexpr int rl = S::radix(S::np - 1, false), Bl = L / rl, Ql = E / rl;
     const int Dy = nlines / Hx;
-    // Green values of the elements this thread multiplies (element e =
-    // threadIdx.x + q * kThreads: kz = e / NL, line = e % NL)
-    R gv[PER];
+    const int ky = l / Hx, kx = l - ky * Hx;
+    const int my = ky <= Dy / 2 ? ky : Dy - ky;
+    R gv[E];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * kThreads;
-        const int p = e / NL, l = l0 + e % NL;
-        gv[q] = R(0);
-        if (e < NL * L && l < nlines) {
-            const int ky = l / Hx, kx = l - ky * Hx;
-            const int my = ky <= Dy / 2 ? ky : Dy - ky;
+    for (int q = 0; q < Ql; ++q)
+#pragma unroll
+        for (int s = 0; s < rl; ++s) {
+            const int p = j + q * TPL + s * Bl;
             const int mz = p <= L / 2 ? p : L - p;
-            gv[q] = ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx];
+            gv[q * rl + s] = live ? ghat[(int64_t(mz) * (gy + 1) + my) * Hx + kx] : R(0);
         }
-    }
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        T2 v;
-        if (p < gz && l < nlines) {
-            v = S[int64_t(p) * nlines + l];
+    // z < gz = L/2 loaded: elements j + s * TPL, s < E/2
+    T2 v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        if (s < E / 2 && live) {
+            v[s] = S1[int64_t(j + s * TPL) * nlines + l];
         } else {
-            v.x = R(0); v.y = R(0);
+            v[s].x = R(0);
+            v[s].y = R(0);
         }
-        buf[LdsLine<L>::at(line, p)] = v;
     }
-    load_twiddles<T2, L>(twl, tw);
-    __syncthreads();
-    lds_fft<T2, L, NL, false>(buf, twl);
-    T2 keep[PER];
+    reg_fft<T2, L, false>(v, lb, j, sw, twl, false);
+    T2 keep[E];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * kThreads;
-        if (e < NL * L) {
-            const int p = e / NL, line = e % NL;
-            T2 v = buf[LdsLine<L>::at(line, p)];
-            v.x = v.x * gv[q];
-            v.y = v.y * gv[q];
-            buf[LdsLine<L>::at(line, p)] = v;
-            keep[q] = v;
-        }
+    for (int e = 0; e < E; ++e) {
+        v[e].x = v[e].x * gv[e];
+        v[e].y = v[e].y * gv[e];
+        keep[e] = v[e];
     }
-    __syncthreads();
-    lds_fft<T2, L, NL, true>(buf, twl);
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        if (p < gz && l < nlines) S[int64_t(p) * nlines + l] = buf[LdsLine<L>::at(line, p)];
-    }
-    __syncthreads();
+    reg_fft<T2, L, true>(v, lb, j, sw, twl, true);
+    if (live) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * kThreads;
-        if (e < NL * L) {
-            const int p = e / NL, line = e % NL;
-            buf[LdsLine<L>::at(line, p)] = mul_minus_i(keep[q], sin_2pi(twl, p, L) * ihz);
-        }
+        for (int s = 0; s < E / 2; ++s) S1[int64_t(j + s * TPL) * nlines + l] = v[s];
     }
-    __syncthreads();
-    lds_fft<T2, L, NL, true>(buf, twl);
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        if (p < gz && l < nlines) S2[int64_t(p) * nlines + l] = buf[LdsLine<L>::at(line, p)];
+#pragma unroll
+    for (int q = 0; q < Ql; ++q)
+#pragma unroll
+        for (int s = 0; s < rl; ++s) {
+            const int p = j + q * TPL + s * Bl;
+            v[q * rl + s] = mul_minus_i(keep[q * rl + s], sin_2pi(twl, p, L) * ihz);
+        }
+    reg_fft<T2, L, true>(v, lb, j, sw, twl, true);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < E / 2; ++s) S2[int64_t(j + s * TPL) * nlines + l] = v[s];
     }
 }
 
 template <typename R, int L>
-__global__ __launch_bounds__(kThreads) void k_y_field(const GridJob* __restrict__ jobs,
-                                                      const typename cplx<R>::type* __restrict__ tw,
-                                                      int nlines, int gy, int Hx)
+__global__ __launch_bounds__((RegLines<L, typename cplx<R>::type>::threads)) void k_y_field_reg(
+    const GridJob* __restrict__ jobs, const typename cplx<R>::type* __restrict__ tw, int nlines,
+    int gy, int Hx)
 {
     using T2 = typename cplx<R>::type;
-    constexpr int NL = FieldLines<L, 3, T2>::value;
-    __shared__ T2 buf[3 * NL * LdsLine<L>::stride];
+    using S = RegPlan<L, T2>;
+    constexpr int NL = RegLines<L, T2>::value;
+    constexpr int E = S::E, TPL = S::TPL;
+    __shared__ T2 buf[NL * L];
     __shared__ T2 twl[L];
     const GridJob& J = jobs[blockIdx.y];
-    T2* __restrict__ S = static_cast<T2*>(J.S);
+    T2* __restrict__ S1 = static_cast<T2*>(J.S);
     T2* __restrict__ S2 = static_cast<T2*>(J.S2);
     const DomainRec* dom = J.dom;
     if (dom->error) return;
     const R ihy = R(dom->inv_h[1]);
-    const int l0 = xcd_group() * NL;
-    const int64_t plane = int64_t(L) * Hx;  // L = Dy rows of Hx
-    for (int e = threadIdx.x; e < NL * L; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        T2 v, vz;
-        if (l < nlines) {
-            int z = l / Hx, c = l - z * Hx;
-            int64_t at = z * plane + int64_t(p) * Hx + c;
-            v = S[at];
-            vz = S2[at];
-        } else {
-            v.x = R(0); v.y = R(0);
-            vz = v;
-        }
-        buf[LdsLine<L>::at(line, p)] = v;
-        buf[LdsLine<L>::at((NL + line), p)] = mul_minus_i(v, sin_2pi(tw, p, L) * ihy);
-        buf[LdsLine<L>::at((2 * NL + line), p)] = vz;
+    const int line = int(threadIdx.x) % NL, j = int(threadIdx.x) / NL;
+    const int l = xcd_group() * NL + line;
+    const bool live = l < nlines;
+    int64_t base = 0;
+    if (live) {
+        const int z = l / Hx, c = l - z * Hx;
+        base = int64_t(z) * L * Hx + c;  // L = Dy rows of Hx
     }
-    load_twiddles<T2, L>(twl, tw);
-    __syncthreads();
-    lds_fft<T2, L, 3 * NL, true>(buf, twl);
-    for (int e = threadIdx.x; e < NL * gy; e += kThreads) {
-        int p = e / NL, line = e % NL;
-        int l = l0 + line;
-        if (l < nlines) {
-            int z = l / Hx, c = l - z * Hx;
-            int64_t at = z * plane + int64_t(p) * Hx + c;
-            S[at] = buf[LdsLine<L>::at(line, p)];
-            S[at + int64_t(gy) * Hx] = buf[LdsLine<L>::at((NL + line), p)];
-            S2[at] = buf[LdsLine<L>::at((2 * NL + line), p)];
+    T2* lb = buf + line * L;
+    const int sw = line & (RegSwz<L, T2>::G - 1);
+    for (int e = threadIdx.x; e < L; e += NL * TPL) twl[e] = tw[e];
+    // inputs of the first inverse pass (radix r0): elements b + s * L / r0
+    constexpr int r0 = S::radix(0, true), B0 = L / r0, Q0 = E / r0;
+    T2 a[E], b[E], c[E];
+#pragma unroll
+    for (int q = 0; q < Q0; ++q)
+#pragma unroll
+        for (int s = 0; s < r0; ++s) {
+            const int p = j + q * TPL + s * B0, e = q * r0 + s;
+            if (live) {
+                a[e] = S1[base + int64_t(p) * Hx];
+                c[e] = S2[base + int64_t(p) * Hx];
+            } else {
+                a[e].x = R(0); a[e].y = R(0);
+                c[e] = a[e];
+            }
+            b[e] = mul_minus_i(a[e], sin_2pi(tw, p, L) * ihy);
         }
+    // rows y < gy kept: the inverse passes end with radix E, elements
+    // j + s * TPL, s < E/2
+    reg_fft<T2, L, true>(a, lb, j, sw, twl, false);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < E / 2; ++s) S1[base + int64_t(j + s * TPL) * Hx] = a[s];
+    }
+    reg_fft<T2, L, true>(b, lb, j, sw, twl, true);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < E / 2; ++s)
+            S1[base + int64_t(gy + j + s * TPL) * Hx] = b[s];
+    }
+    reg_fft<T2, L, true>(c, lb, j, sw, twl, true);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < E / 2; ++s) S2[base + int64_t(j + s * TPL) * Hx] = c[s];
     }
 }
 
@@ -938,10 +1145,10 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 0>),
-                               dim3(col_blocks(lines, FieldLines<L, 1, T2>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
-                               int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.gy, g.Dy, g.gy, g.Hx);
+            hipLaunchKernelGGL((k_cols_reg<R, L, 0>),
+                               dim3(col_blocks(lines, RegLines<L, T2>::value), nj),
+                               dim3(RegLines<L, T2>::threads), 0, st, jobs, tw + twiddle_offset(L),
+                               lines, g.Hx, int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.gy, g.Hx);
         }
     };
     template <int L> struct YInv {
@@ -949,10 +1156,10 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 1>),
-                               dim3(col_blocks(lines, FieldLines<L, 1, T2>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.Hx,
-                               int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.Dy, g.gy, g.gy, g.Hx);
+            hipLaunchKernelGGL((k_cols_reg<R, L, 1>),
+                               dim3(col_blocks(lines, RegLines<L, T2>::value), nj),
+                               dim3(RegLines<L, T2>::threads), 0, st, jobs, tw + twiddle_offset(L),
+                               lines, g.Hx, int64_t(g.Dy) * g.Hx, int64_t(g.Hx), g.gy, g.Hx);
         }
     };
     template <int L> struct ZConv {
@@ -960,10 +1167,10 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.Dy * g.Hx;
-            hipLaunchKernelGGL((k_cols<R, L, 2>),
-                               dim3(col_blocks(lines, FieldLines<L, 1, T2>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, lines,
-                               int64_t(0), int64_t(lines), g.gz, g.gz, g.gy, g.Hx);
+            hipLaunchKernelGGL((k_cols_reg<R, L, 2>),
+                               dim3(col_blocks(lines, RegLines<L, T2>::value), nj),
+                               dim3(RegLines<L, T2>::threads), 0, st, jobs, tw + twiddle_offset(L),
+                               lines, lines, int64_t(0), int64_t(lines), g.gy, g.Hx);
         }
     };
     template <int L> struct ZField {
@@ -971,10 +1178,10 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.Dy * g.Hx;
-            hipLaunchKernelGGL((k_z_field<R, L>),
-                               dim3(col_blocks(lines, ZFieldLines<L, T2>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gz,
-                               g.gy, g.Hx);
+            hipLaunchKernelGGL((k_z_field_reg<R, L>),
+                               dim3(col_blocks(lines, RegLines<L, T2>::value), nj),
+                               dim3(RegLines<L, T2>::threads), 0, st, jobs, tw + twiddle_offset(L),
+                               lines, g.gy, g.Hx);
         }
     };
     template <int L> struct YField {
@@ -982,10 +1189,10 @@ template <typename R> struct SolveX {
                         hipStream_t st)
         {
             int lines = g.gz * g.Hx;
-            hipLaunchKernelGGL((k_y_field<R, L>),
-                               dim3(col_blocks(lines, FieldLines<L, 3, T2>::value), nj),
-                               dim3(kThreads), 0, st, jobs, tw + twiddle_offset(L), lines, g.gy,
-                               g.Hx);
+            hipLaunchKernelGGL((k_y_field_reg<R, L>),
+                               dim3(col_blocks(lines, RegLines<L, T2>::value), nj),
+                               dim3(RegLines<L, T2>::threads), 0, st, jobs, tw + twiddle_offset(L),
+                               lines, g.gy, g.Hx);
         }
     };
     template <int M> struct XField {

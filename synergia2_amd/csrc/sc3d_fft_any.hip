@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void ga_x_c2r(const GridJob* __restrict__
     }
 }
 
-// Column passes over S, as k_cols: line l -> (plane = l / lpp, c = l % lpp),
+// Column passes over S, as k_cols_reg: line l -> (plane = l / lpp, c = l % lpp),
 // element p at S[plane*plane_stride + p*pstride + c]; loads p < nload,
 // stores p < nstore.  MODE 0 forward, 1 inverse, 2 forward x Green's
 // spectrum, inverse (the z pass).
@@ -271,8 +271,8 @@ __global__ __launch_bounds__(kThreads) void ga_cols(const GridJob* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Precision form: the field straight from the spectrum, as k_z_field /
-// k_y_field / k_x_field (D_a(k) = -i sin(2 pi k / D_a) / h_a).
+// Precision form: the field straight from the spectrum, as k_z_field_reg /
+// k_y_field_reg / k_x_field (D_a(k) = -i sin(2 pi k / D_a) / h_a).
 // z : per (ky, kx) column: forward, x Ghat, inverse of T and of T*D_z,
 //     keep z < gz: T -> S, T_z -> S2
 // ---------------------------------------------------------------------------
