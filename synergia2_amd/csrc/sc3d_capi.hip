@@ -171,7 +171,7 @@ int64_t ctx_footprint(const sc3d_options& o, int64_t n)
     int64_t b = sizeof(DomainRec) + sizeof(GridJob) + sizeof(StepCtl);
     b += int64_t(np) * sizeof(GreenPatch);
     if (spectral) b += int64_t(g.Dz) * g.Dy * (g.Dx + 2 * g.Hx) * rs;  // C + chat
-    b += g.G * rs;                               // rho
+    b += (g.G * rs + 15) / 16 * 16;              // rho
     b += g.S_elems * 2 * rs * (fp32 ? 2 : 1);    // S (+ S2)
     b += fp32 ? 0 : g.G * rs;                    // phi
     b += g.G * 4 * rs;                           // E
@@ -250,7 +250,8 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
         (sc.ctl != nullptr ? sc.sort_now != 0 : (c->applies % o.sort_period) == 0);
     const bool do_sort = n > 0 && (do_reorder || !local_dep);
     ev[0] = mark(c, st);
-    launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, sc, st);
+    launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, sc, c->rho, c->rho_bytes,
+                  st);
     ev[1] = mark(c, st);
     const int32_t* dep_perm = nullptr;
     if (do_sort) {
@@ -261,14 +262,11 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
                            c->dom, st);
         if (!local_dep) dep_perm = c->sb.perm;
     }
+    ev[2] = mark(c, st);
     if (!c->fp32) {
-        launch_fill<double>(static_cast<double*>(c->rho), g.G, st);
-        ev[2] = mark(c, st);
         launch_deposit<double>(b->parts, b->stride, b->masks, dep_perm, local_dep, n, g.gx, g.gy,
                                g.gz, c->dom, static_cast<double*>(c->rho), st);
     } else {
-        launch_fill<float>(static_cast<float*>(c->rho), g.G, st);
-        ev[2] = mark(c, st);
         launch_deposit<float>(b->parts, b->stride, b->masks, dep_perm, local_dep, n, g.gx, g.gy,
                               g.gz, c->dom, static_cast<float*>(c->rho), st);
     }
@@ -388,7 +386,8 @@ int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)
     int rc = 0;
     rc |= dmalloc(c, &c->dom, 1);
     char* p = nullptr;
-    rc |= dmalloc(c, &p, g.G * rs);
+    c->rho_bytes = (g.G * rs + 15) / 16 * 16;
+    rc |= dmalloc(c, &p, c->rho_bytes);
     c->rho = p;
     rc |= dmalloc(c, &p, g.S_elems * 2 * rs);
     c->S = p;

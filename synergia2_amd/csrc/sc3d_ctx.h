@@ -19,6 +19,7 @@ struct sc3d_ctx {
     double* P = nullptr;
     int64_t P_cap = 0;
     void* rho = nullptr;
+    int64_t rho_bytes = 0;  // the density grid, rounded up to 16 bytes
     void* S = nullptr;
     void* S2 = nullptr;   // precision form: second spectrum workspace
     void* phi = nullptr;  // FP64 form only

@@ -52,7 +52,7 @@ constexpr int kSortTileItems = 4096;
 
 void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                    double* P, const DomainFinishArgs& a, DomainRec* dom, const Sched& sc,
-                   hipStream_t st);
+                   void* grid, int64_t grid_bytes, hipStream_t st);
 void launch_step_begin(StepCtl* ctl, hipStream_t st);
 // scheduled maps before the kick (the first step's first half-step)
 void launch_maps_sched(double* parts, int64_t stride, const uint8_t* masks, int64_t n,
@@ -62,8 +62,6 @@ void launch_sort(const double* parts, int64_t stride, const uint8_t* masks, int6
                  int gy, int gz, const DomainRec* dom, SortBuffers& b, hipStream_t st);
 void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, const int32_t* perm,
                     double* scratch, uint8_t* smask, const DomainRec* dom, hipStream_t st);
-template <typename R>
-void launch_fill(R* p, int64_t n, hipStream_t st);
 // Deposit: local (k_deposit_tile): windows of the bunch storage through an
 // LDS node tile of each window's cell box (the storage is cell-sorted at
 // every reorder); else (k_deposit): through the permutation `perm` of this

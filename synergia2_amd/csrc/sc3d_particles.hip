@@ -192,48 +192,6 @@ void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, i
                        out + 6);
 }
 
-__global__ __launch_bounds__(kThreads) void k_domain_partial(const double* __restrict__ parts,
-                                                             int64_t stride,
-                                                             const uint8_t* __restrict__ masks,
-                                                             int64_t n, double* __restrict__ P,
-                                                             const StepCtl* __restrict__ ctl)
-{
-    __shared__ double ws[4][6];
-    if (ctl && ctl->done) return;
-    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int64_t i0 = int64_t(blockIdx.x) * 4096 + threadIdx.x;
-    // All 16 rounds' loads first (none waits for a mask), then the sums in the
-    // fixed order (measured at 1.6e7 particles: 108 -> 103 us per step for the
-    // domain).
-    double x[16], y[16], z[16];
-    uint8_t on[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int64_t i = i0 + int64_t(j) * 256;
-        on[j] = 0;
-        if (i < n) {
-            on[j] = masks[i];  // tested only after every load is issued
-            x[j] = parts[i];
-            y[j] = parts[i + 2 * stride];
-            z[j] = parts[i + 4 * stride];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        if (on[j]) {
-            s[0] += x[j];
-            s[1] += y[j];
-            s[2] += z[j];
-            s[3] += x[j] * x[j];
-            s[4] += y[j] * y[j];
-            s[5] += z[j] * z[j];
-        }
-    }
-    block_pairwise6(s, ws);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 6; ++q) P[int64_t(blockIdx.x) * 6 + q] = s[q];
-}
-
 // space_charge_3d_open_hockney.cc:94-109
 __device__ double smallest_non_tiny(double val, double o1, double o2, double tiny)
 {
@@ -347,6 +305,59 @@ __global__ __launch_bounds__(kThreads) void k_domain_finish(const double* __rest
     for (int k = 0; k < 3; ++k) d.ghat_h[k] = h[k];
     d.ghat_valid = 1;
     *dom = d;
+}
+
+// The chunk sums and the clearing of the density grid in one launch: every
+// block clears its slice of the grid (16-byte stores; the grid of the
+// previous apply was consumed by its solve) and writes its chunk partials.
+// (Finishing in the last block to count itself in, behind device-scope
+// fences, measured slower: 41 against 25 us per apply at 4e6 particles, the
+// fence of each of the 977 blocks writing back its L2.)
+__global__ __launch_bounds__(kThreads) void k_domain_partial(
+    const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
+    int64_t n, double* __restrict__ P, const StepCtl* __restrict__ ctl, uint4* __restrict__ grid,
+    int64_t grid16)
+{
+    __shared__ double ws[4][6];
+    if (ctl && ctl->done) return;
+    if (grid16 > 0) {
+        const int64_t per = (grid16 + gridDim.x - 1) / gridDim.x;
+        const int64_t g0 = int64_t(blockIdx.x) * per;
+        const int64_t g1 = g0 + per < grid16 ? g0 + per : grid16;
+        for (int64_t i = g0 + threadIdx.x; i < g1; i += kThreads) grid[i] = make_uint4(0, 0, 0, 0);
+    }
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t i0 = int64_t(blockIdx.x) * 4096 + threadIdx.x;
+    // All 16 rounds' loads first (none waits for a mask), then the sums in the
+    // fixed order (measured at 1.6e7 particles: 108 -> 103 us per step for the
+    // domain).
+    double x[16], y[16], z[16];
+    uint8_t on[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int64_t i = i0 + int64_t(j) * 256;
+        on[j] = 0;
+        if (i < n) {
+            on[j] = masks[i];  // tested only after every load is issued
+            x[j] = parts[i];
+            y[j] = parts[i + 2 * stride];
+            z[j] = parts[i + 4 * stride];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (on[j]) {
+            s[0] += x[j];
+            s[1] += y[j];
+            s[2] += z[j];
+            s[3] += x[j] * x[j];
+            s[4] += y[j] * y[j];
+            s[5] += z[j] * z[j];
+        }
+    }
+    block_pairwise6(s, ws);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) P[int64_t(blockIdx.x) * 6 + q] = s[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1621,15 +1632,25 @@ void launch_adjust(double* parts, int64_t stride, int64_t n, const AdjustArgs& a
                            a);
 }
 
+// The density grid (grid_bytes, a multiple of 16) is cleared here: by the
+// chunk pass when its chunks are enough for slices of at most 32 KiB each,
+// else by a fill of its own.
 void launch_domain(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                    double* P, const DomainFinishArgs& a, DomainRec* dom, const Sched& sc,
-                   hipStream_t st)
+                   void* grid, int64_t grid_bytes, hipStream_t st)
 {
-    int64_t nch = (n + 4095) / 4096;
-    if (!a.fixed && nch > 0)
+    const int64_t nch = (n + 4095) / 4096;
+    const int64_t grid16 = grid_bytes / 16;
+    const bool chunks = !a.fixed && nch > 0;
+    const bool clear = chunks && grid_bytes <= nch * 32768;
+    if (chunks)
         hipLaunchKernelGGL(k_domain_partial, dim3(unsigned(nch)), dim3(kThreads), 0, st, parts,
-                           stride, masks, n, P, sc.ctl);
-    hipLaunchKernelGGL(k_domain_finish, dim3(1), dim3(kThreads), 0, st, P, a.fixed ? 0 : nch, a,
+                           stride, masks, n, P, sc.ctl, static_cast<uint4*>(grid),
+                           clear ? grid16 : int64_t(0));
+    if (!clear)
+        hipLaunchKernelGGL(k_fill<uint4>, dim3(grid_for(grid16, 2048)), dim3(kThreads), 0, st,
+                           static_cast<uint4*>(grid), grid16, make_uint4(0, 0, 0, 0));
+    hipLaunchKernelGGL(k_domain_finish, dim3(1), dim3(kThreads), 0, st, P, chunks ? nch : 0, a,
                        dom, sc);
 }
 
@@ -1698,12 +1719,6 @@ void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, co
                        perm, n, scratch, smask, dom);
     hipLaunchKernelGGL(k_copy_back, grid, dim3(kThreads), 0, st, parts, stride, masks, n,
                        scratch, smask, dom);
-}
-
-template <typename R>
-void launch_fill(R* p, int64_t n, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_fill<R>, dim3(grid_for(n, 2048)), dim3(kThreads), 0, st, p, n, R(0));
 }
 
 // Particles per lane of the deposit: 4 on dense bunches, 2 on sparse ones.
@@ -1795,8 +1810,6 @@ void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
                        masks, a);
 }
 
-template void launch_fill<double>(double*, int64_t, hipStream_t);
-template void launch_fill<float>(float*, int64_t, hipStream_t);
 template void launch_deposit<double>(const double*, int64_t, const uint8_t*, const int32_t*,
                                      bool, int64_t, int, int, int, DomainRec*, double*,
                                      hipStream_t);
