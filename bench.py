@@ -86,9 +86,9 @@ def make_workload(sc, S, config, precision=None, sort_period=None):
 
 def algorithmic_bytes(phase, n, g, w):
     """Algorithmic HBM bytes of one launch (unit: one particle or one node):
-    k_kick   73 B/particle (x,y,cdt,xp,yp,dpop read, xp,yp,dpop written, mask)
+    kick     73 B/particle (x,y,cdt,xp,yp,dpop read, xp,yp,dpop written, mask)
              + 3 field words per node read once;
-    k_deposit 25 B/particle (x,y,cdt, mask) + 1 density word per node written."""
+    deposit  25 B/particle (x,y,cdt, mask) + 1 density word per node written."""
     G = g[0] * g[1] * g[2]
     if phase == "sc3d_kick":
         return 73 * n + 3 * w * G
@@ -247,7 +247,7 @@ def timed_sc_run(args, sc, S, config, steps, warmup):
     w = 8 if precision == 0 else 4
     traffic = measured_traffic(config)
     lines = []
-    for phase, kname in (("sc3d_kick", "k_kick"), ("sc3d_local_rho", "k_deposit")):
+    for phase, kname in (("sc3d_kick", "k_kick_tile"), ("sc3d_local_rho", "k_deposit_tile")):
         per_launch_s = phases[phase] / steps / 1e3
         B = algorithmic_bytes(phase, n, grid, w)
         achieved = B / per_launch_s / 1e9
@@ -668,9 +668,10 @@ def accuracy_report(sc, S, config, steps):
 
 
 def measured_traffic(config):
-    """HBM bytes per launch of k_kick / k_deposit from the committed
-    rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this workload (calibrated
-    against a copy of known size), or {} when none was taken for it."""
+    """HBM bytes per launch of the kick (the map-free k_kick_tile) and of
+    k_deposit_tile from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE runs
+    of this workload (calibrated against a copy of known size), or {} when
+    none was taken for it."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_sc_step_{config}.json")
     if not os.path.exists(path):
         return {}
@@ -678,10 +679,10 @@ def measured_traffic(config):
         doc = json.load(f)
     out = {}
     for name, rec in doc["kernels"].items():
-        if "k_kick<double, false>" in name or "k_kick<float, false>" in name:
-            out["k_kick"] = rec["traffic_bytes"]
-        elif "k_deposit" in name:
-            out["k_deposit"] = rec["traffic_bytes"]
+        if "k_kick_tile<" in name:
+            out["k_kick_tile"] = rec["traffic_bytes"]
+        elif "k_deposit_tile<" in name:
+            out["k_deposit_tile"] = rec["traffic_bytes"]
     return out
 
 
