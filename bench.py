@@ -117,6 +117,8 @@ def main():
                     help="measure the served form on the Poisson job stream instead (see "
                          "served_bench)")
     ap.add_argument("--jobs", type=int, default=256, help="served: jobs in the stream")
+    ap.add_argument("--branches", type=int, default=8,
+                    help="served: streams the per-job chains are spread over")
     ap.add_argument("--form", choices=("served", "plain64", "plain32"), default=None,
                     help="run only one form on the job stream, all jobs at once (served) or "
                          "one after another through the operator API (plain64 / plain32), and "
@@ -411,7 +413,7 @@ def served_record(args, sc, S):
           f"{stream[-1][0]:.2f} s", file=sys.stderr, flush=True)
 
     def drive(real_time):
-        srv = SV.Server(SV.options())
+        srv = SV.Server(SV.options(branches=args.branches))
         pending = list(stream)
         ids = []
         t0 = time.perf_counter()
@@ -488,7 +490,7 @@ def form_run(args, sc, S):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if args.form == "served":
-        srv = SV.Server(SV.options())
+        srv = SV.Server(SV.options(branches=args.branches))
         for _, kw in stream:
             srv.submit(SV.Job(**kw))
         while True:

@@ -75,7 +75,7 @@ typedef struct sc3d_server_options {
     int32_t admit_interval;   /* with graphs: admit at most every this many   */
                               /* iterations while jobs are resident (0: 8)    */
     int32_t branches;         /* streams the per-job chains are spread over   */
-                              /* (independent graph branches; 0: 4)           */
+                              /* (independent graph branches; 0: 8)           */
     int32_t keep_particles;   /* copy each finished job's particles to host   */
                               /* memory (sc3d_server_job_particles)           */
 } sc3d_server_options;

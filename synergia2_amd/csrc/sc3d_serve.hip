@@ -389,7 +389,7 @@ void sc3d_server_options_default(sc3d_server_options* o)
     o->use_graphs = 1;
     o->batch_grids = 1;
     o->admit_interval = 8;
-    o->branches = 4;
+    o->branches = 8;
 }
 
 int sc3d_server_create(const sc3d_server_options* opt, sc3d_server** out)
@@ -404,7 +404,7 @@ int sc3d_server_create(const sc3d_server_options* opt, sc3d_server** out)
     auto s = std::make_unique<sc3d_server>();
     s->opt = *opt;
     if (s->opt.admit_interval == 0) s->opt.admit_interval = 8;
-    if (s->opt.branches == 0) s->opt.branches = 4;
+    if (s->opt.branches == 0) s->opt.branches = 8;
     HIP_TRY(hipGetDevice(&s->device));
     if (s->opt.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
