@@ -263,12 +263,18 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
         if (!local_dep) dep_perm = c->sb.perm;
     }
     ev[2] = mark(c, st);
+    // after a reorder the cell-sorted bunch is the gathered copy until the
+    // kick writes it back
+    c->reordering = do_reorder;
+    const double* dp = do_reorder ? c->scratch : b->parts;
+    const int64_t ds = do_reorder ? n : b->stride;
+    const uint8_t* dm = do_reorder ? c->smask : b->masks;
     if (!c->fp32) {
-        launch_deposit<double>(b->parts, b->stride, b->masks, dep_perm, local_dep, n, g.gx, g.gy,
-                               g.gz, c->dom, static_cast<double*>(c->rho), st);
+        launch_deposit<double>(dp, ds, dm, dep_perm, local_dep, n, g.gx, g.gy, g.gz, c->dom,
+                               static_cast<double*>(c->rho), st);
     } else {
-        launch_deposit<float>(b->parts, b->stride, b->masks, dep_perm, local_dep, n, g.gx, g.gy,
-                              g.gz, c->dom, static_cast<float*>(c->rho), st);
+        launch_deposit<float>(dp, ds, dm, dep_perm, local_dep, n, g.gx, g.gy, g.gz, c->dom,
+                              static_cast<float*>(c->rho), st);
     }
     ev[3] = mark(c, st);
     return SC3D_OK;
@@ -311,12 +317,17 @@ void enqueue_kick(sc3d_ctx* c, const sc3d_bunch* b, const Sched& sc, hipStream_t
     MapProgram post = c->post;
     post.ref_p = b->ref_p;
     post.mass = b->mass;
+    const double* src = c->reordering ? c->scratch : nullptr;
+    const uint8_t* smasks = c->reordering ? c->smask : nullptr;
+    c->reordering = false;
     if (!c->fp32)
         launch_kick<double>(b->parts, b->stride, b->masks, b->n_active, g.gx, g.gy, g.gz, c->dom,
-                            static_cast<double4*>(c->E), b->ref_p, b->mass, post, sc, st);
+                            static_cast<double4*>(c->E), b->ref_p, b->mass, post, sc, src,
+                            smasks, st);
     else
         launch_kick<float>(b->parts, b->stride, b->masks, b->n_active, g.gx, g.gy, g.gz, c->dom,
-                           static_cast<float4*>(c->E), b->ref_p, b->mass, post, sc, st);
+                           static_cast<float4*>(c->E), b->ref_p, b->mass, post, sc, src, smasks,
+                           st);
 }
 
 }  // namespace sc3d

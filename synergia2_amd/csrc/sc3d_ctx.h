@@ -34,6 +34,9 @@ struct sc3d_ctx {
     sc3d::SortBuffers sb{};
     double* scratch = nullptr;
     uint8_t* smask = nullptr;
+    // the apply being enqueued reorders the bunch: its deposit reads the
+    // gathered copy in scratch / smask, and its kick writes it back
+    bool reordering = false;
     int64_t applies = 0;
     int64_t sorted_n = -1;
     hipStream_t last_stream = nullptr;

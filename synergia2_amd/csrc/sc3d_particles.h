@@ -78,10 +78,14 @@ void launch_deposit<float>(const double* parts, int64_t stride, const uint8_t* m
 template <typename R>
 void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
                    typename vec4<R>::type* E, hipStream_t st);
+// Kick (+ maps): src / smasks non-null on the apply that reordered the bunch:
+// the particles are read from the reorder's gathered copy (7 columns of
+// stride n) and written whole to the storage (the reorder's copy back).
 template <typename R>
-void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
-                 int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
-                 double m, const MapProgram& prog, const Sched& sc, hipStream_t st);
+void launch_kick(double* parts, int64_t stride, uint8_t* masks, int64_t n, int gx, int gy, int gz,
+                 const DomainRec* dom, const typename vec4<R>::type* E, double pref, double m,
+                 const MapProgram& prog, const Sched& sc, const double* src,
+                 const uint8_t* smasks, hipStream_t st);
 void launch_moments(const double* parts, int64_t stride, const uint8_t* masks, int64_t n,
                     double total_num, double* P, double* out, hipStream_t st);
 struct TailArgs {

@@ -675,7 +675,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     }
 }
 
-// Reorder: scratch[j] = bunch[perm[j]] (7 columns + mask), then copy back.
+// Reorder: scratch[j] = bunch[perm[j]] (7 columns + mask); the apply's deposit reads
+// scratch and its kick writes it back to the bunch storage.
 __global__ __launch_bounds__(kThreads) void k_gather_particles(
     const double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks,
     const int32_t* __restrict__ perm, int64_t n, double* __restrict__ scratch,
@@ -689,21 +690,6 @@ __global__ __launch_bounds__(kThreads) void k_gather_particles(
 #pragma unroll
         for (int c = 0; c < 7; ++c) scratch[j + c * n] = parts[src + c * stride];
         smask[j] = masks[src];
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void k_copy_back(double* __restrict__ parts, int64_t stride,
-                                                        uint8_t* __restrict__ masks, int64_t n,
-                                                        const double* __restrict__ scratch,
-                                                        const uint8_t* __restrict__ smask,
-                                                        const DomainRec* dom)
-{
-    if (dom->error) return;
-    for (int64_t j = int64_t(part_block()) * kThreads + threadIdx.x; j < n;
-         j += int64_t(gridDim.x) * kThreads) {
-#pragma unroll
-        for (int c = 0; c < 7; ++c) parts[j + c * stride] = scratch[j + c * n];
-        masks[j] = smask[j];
     }
 }
 
@@ -1352,6 +1338,10 @@ __global__ __launch_bounds__(kThreads) void k_efield(const R* __restrict__ phi, 
 // MAPS: the map program that follows the kick in the split-operator
 // sequence is applied to every active particle in the same pass (one read
 // and one write of the six coordinates for kick + maps).
+// COPY (the apply that reordered the bunch): the particles are read from the
+// cell-sorted copy the reorder gathered (src: 7 columns of stride n, and its
+// masks), and every particle's 7 columns and mask are written to the bunch
+// storage -- the reorder's copy back, folded into the kick.
 // ---------------------------------------------------------------------------
 // Kick factor: the FP64 form's is the record's; the precision form's grid
 // holds unit-weight charge scaled to unit peak, so w0 * peak returns here.
@@ -1363,14 +1353,15 @@ __device__ __forceinline__ double kick_factor(const DomainRec& d)
     return d.factor * d.w0 * (peak > 0.0f ? double(peak) : 1.0);
 }
 
-template <typename R, bool MAPS>
+template <typename R, bool MAPS, bool COPY>
 __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, int64_t stride,
-                                                   const uint8_t* __restrict__ masks, int64_t n,
+                                                   uint8_t* __restrict__ masks, int64_t n,
                                                    int gx, int gy, int gz,
                                                    const DomainRec* __restrict__ dom,
                                                    const typename vec4<R>::type* __restrict__ E,
                                                    double pref, double m, MapProgram prog,
-                                                   Sched sc)
+                                                   Sched sc, const double* __restrict__ src,
+                                                   const uint8_t* __restrict__ smasks)
 {
     if (dom->error) return;
     // served form: the program fused after the kick comes from the schedule
@@ -1379,19 +1370,26 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
     const double lx = dom->left[0], ly = dom->left[1], lz = dom->left[2];
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
     const double factor = kick_factor<R>(*dom);
+    const double* __restrict__ rd = COPY ? src : parts;
+    const int64_t rs = COPY ? n : stride;
     for (int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * kThreads) {
-        if (!masks[i]) continue;
+        const uint8_t on = COPY ? smasks[i] : masks[i];
+        if (!COPY && !on) continue;
         // all six coordinates are loaded up front (independent loads in flight)
-        double x = parts[i], y = parts[i + 2 * stride], cdt = parts[i + 4 * stride];
-        double xp = parts[i + stride], yp = parts[i + 3 * stride], dpop = parts[i + 5 * stride];
+        double x = rd[i], y = rd[i + 2 * rs], cdt = rd[i + 4 * rs];
+        double xp = rd[i + rs], yp = rd[i + 3 * rs], dpop = rd[i + 5 * rs];
         double ox, oy, oz;
         const int ix = cell_index(x, lx, ihx, ox);
         const int iy = cell_index(y, ly, ihy, oy);
         const int iz = cell_index(cdt, lz, ihz, oz);
-        const bool in = (ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) &&
+        const bool in = on && (ix >= 0 && ix < gx - 1) && (iy >= 0 && iy < gy - 1) &&
                         (iz >= 0 && iz < gz - 1);
-        if (!MAPS && !in) continue;
+        if (COPY) {
+            masks[i] = on;
+            parts[i + 6 * stride] = rd[i + 6 * rs];
+        }
+        if (!MAPS && !COPY && !in) continue;
         if (in) {
             const double aox = 1.0 - ox, aoy = 1.0 - oy, aoz = 1.0 - oz;
             const int64_t b0 = (int64_t(iz) * gy + iy) * gx + ix;
@@ -1415,7 +1413,7 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
             yp += factor * vy;
             xp += factor * vx;
         }
-        if (MAPS) {
+        if (MAPS && on) {
             if (sp) {
                 for (int k = 0; k < sp->n; ++k)
                     apply_slice(sp->s[k], x, xp, y, yp, cdt, dpop, sp->ref_p, sp->mass);
@@ -1423,6 +1421,8 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
                 for (int k = 0; k < prog.n; ++k)
                     apply_slice(prog.s[k], x, xp, y, yp, cdt, dpop, prog.ref_p, prog.mass);
             }
+        }
+        if (MAPS || COPY) {
             parts[i] = x;
             parts[i + 2 * stride] = y;
             parts[i + 4 * stride] = cdt;
@@ -1451,11 +1451,11 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ parts, i
 constexpr int kKickPer = 2;
 constexpr int kKickNodes = 1024;
 
-template <typename R>
+template <typename R, bool COPY>
 __global__ __launch_bounds__(kThreads) void k_kick_tile(
-    double* __restrict__ parts, int64_t stride, const uint8_t* __restrict__ masks, int64_t n,
-    int gx, int gy, int gz, const DomainRec* __restrict__ dom,
-    const typename vec4<R>::type* __restrict__ E, double pref, double m)
+    double* __restrict__ parts, int64_t stride, uint8_t* __restrict__ masks, int64_t n, int gx,
+    int gy, int gz, const DomainRec* __restrict__ dom, const typename vec4<R>::type* __restrict__ E,
+    double pref, double m, const double* __restrict__ src, const uint8_t* __restrict__ smasks)
 {
     __shared__ R tile[3][kKickNodes];
     __shared__ int box[kThreads / 64][6];
@@ -1465,6 +1465,8 @@ __global__ __launch_bounds__(kThreads) void k_kick_tile(
     const double factor = kick_factor<R>(*dom);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t base = int64_t(blockIdx.x) * (kThreads * kKickPer) + t;
+    const double* __restrict__ rd = COPY ? src : parts;
+    const int64_t rs = COPY ? n : stride;
     double x[kKickPer], y[kKickPer], cdt[kKickPer], xp[kKickPer], yp[kKickPer], dpop[kKickPer];
     double ox[kKickPer], oy[kKickPer], oz[kKickPer];
     int ix[kKickPer], iy[kKickPer], iz[kKickPer];
@@ -1476,13 +1478,21 @@ __global__ __launch_bounds__(kThreads) void k_kick_tile(
         act[q] = false;
         x[q] = y[q] = cdt[q] = xp[q] = yp[q] = dpop[q] = 0.0;
         if (i < n) {
-            act[q] = masks[i] != 0;
-            x[q] = parts[i];
-            y[q] = parts[i + 2 * stride];
-            cdt[q] = parts[i + 4 * stride];
-            xp[q] = parts[i + stride];
-            yp[q] = parts[i + 3 * stride];
-            dpop[q] = parts[i + 5 * stride];
+            const uint8_t on = COPY ? smasks[i] : masks[i];
+            act[q] = on != 0;
+            x[q] = rd[i];
+            y[q] = rd[i + 2 * rs];
+            cdt[q] = rd[i + 4 * rs];
+            xp[q] = rd[i + rs];
+            yp[q] = rd[i + 3 * rs];
+            dpop[q] = rd[i + 5 * rs];
+            if (COPY) {
+                masks[i] = on;
+                parts[i] = x[q];
+                parts[i + 2 * stride] = y[q];
+                parts[i + 4 * stride] = cdt[q];
+                parts[i + 6 * stride] = rd[i + 6 * rs];
+            }
         }
         ix[q] = cell_index(x[q], lx, ihx, ox[q]);
         iy[q] = cell_index(y[q], ly, ihy, oy[q]);
@@ -1535,8 +1545,8 @@ __global__ __launch_bounds__(kThreads) void k_kick_tile(
     }
 #pragma unroll
     for (int q = 0; q < kKickPer; ++q) {
-        if (!in[q]) continue;
-        {
+        if (COPY ? base + int64_t(q) * kThreads >= n : !in[q]) continue;
+        if (in[q]) {
             const double aox = 1.0 - ox[q], aoy = 1.0 - oy[q], aoz = 1.0 - oz[q];
             R ex[8], ey[8], ez[8];
             if (staged) {
@@ -1868,11 +1878,11 @@ void launch_reorder(double* parts, int64_t stride, uint8_t* masks, int64_t n, co
     // One particle per thread up to 1.7e7 particles: the gather is bound by the latency of its
     // random reads, so more waves in flight help (measured at 1.6e7 particles, one sort every
     // 12 steps: sort plus reorder 132 us per step on average with 65536 blocks, 138 with 8192).
+    // The copy of the gathered bunch back into the storage is the kick's (k_kick*<.., COPY>):
+    // until then the deposit reads the gathered copy.
     const dim3 grid(grid_for(n, 65536));
     hipLaunchKernelGGL(k_gather_particles, grid, dim3(kThreads), 0, st, parts, stride, masks,
                        perm, n, scratch, smask, dom);
-    hipLaunchKernelGGL(k_copy_back, grid, dim3(kThreads), 0, st, parts, stride, masks, n,
-                       scratch, smask, dom);
 }
 
 // Particles per lane of the deposit: 4 on dense bunches, 2 on sparse ones.
@@ -1942,22 +1952,34 @@ void launch_efield(const R* phi, int gx, int gy, int gz, const DomainRec* dom,
 }
 
 template <typename R>
-void launch_kick(double* parts, int64_t stride, const uint8_t* masks, int64_t n, int gx, int gy,
-                 int gz, const DomainRec* dom, const typename vec4<R>::type* E, double pref,
-                 double m, const MapProgram& prog, const Sched& sc, hipStream_t st)
+void launch_kick(double* parts, int64_t stride, uint8_t* masks, int64_t n, int gx, int gy, int gz,
+                 const DomainRec* dom, const typename vec4<R>::type* E, double pref, double m,
+                 const MapProgram& prog, const Sched& sc, const double* src,
+                 const uint8_t* smasks, hipStream_t st)
 {
     if (n <= 0) return;
+    const bool copy = src != nullptr;
     if (prog.n == 0 && !sc.progs) {
         constexpr int win = kThreads * kKickPer;
-        hipLaunchKernelGGL(k_kick_tile<R>, dim3(unsigned((n + win - 1) / win)), dim3(kThreads), 0,
-                           st, parts, stride, masks, n, gx, gy, gz, dom, E, pref, m);
+        const dim3 grid(unsigned((n + win - 1) / win));
+        if (copy)
+            hipLaunchKernelGGL((k_kick_tile<R, true>), grid, dim3(kThreads), 0, st, parts, stride,
+                               masks, n, gx, gy, gz, dom, E, pref, m, src, smasks);
+        else
+            hipLaunchKernelGGL((k_kick_tile<R, false>), grid, dim3(kThreads), 0, st, parts,
+                               stride, masks, n, gx, gy, gz, dom, E, pref, m, src, smasks);
         return;
     }
     // kick + maps: up to 32768 blocks (measured at 1.6e7 particles: 240 us,
     // against 251 with 8192 blocks, 272 with 4096 and 247 with one particle
     // per thread, for the kick alone)
-    hipLaunchKernelGGL((k_kick<R, true>), dim3(grid_for(n, 32768)), dim3(kThreads), 0, st, parts,
-                       stride, masks, n, gx, gy, gz, dom, E, pref, m, prog, sc);
+    const dim3 grid(grid_for(n, 32768));
+    if (copy)
+        hipLaunchKernelGGL((k_kick<R, true, true>), grid, dim3(kThreads), 0, st, parts, stride,
+                           masks, n, gx, gy, gz, dom, E, pref, m, prog, sc, src, smasks);
+    else
+        hipLaunchKernelGGL((k_kick<R, true, false>), grid, dim3(kThreads), 0, st, parts, stride,
+                           masks, n, gx, gy, gz, dom, E, pref, m, prog, sc, src, smasks);
 }
 
 void launch_populate(double* parts, int64_t stride, int64_t n, uint8_t* masks,
@@ -1974,11 +1996,13 @@ template void launch_efield<double>(const double*, int, int, int, const DomainRe
                                     hipStream_t);
 template void launch_efield<float>(const float*, int, int, int, const DomainRec*, float4*,
                                    hipStream_t);
-template void launch_kick<double>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
+template void launch_kick<double>(double*, int64_t, uint8_t*, int64_t, int, int, int,
                                   const DomainRec*, const double4*, double, double,
-                                  const MapProgram&, const Sched&, hipStream_t);
-template void launch_kick<float>(double*, int64_t, const uint8_t*, int64_t, int, int, int,
+                                  const MapProgram&, const Sched&, const double*, const uint8_t*,
+                                  hipStream_t);
+template void launch_kick<float>(double*, int64_t, uint8_t*, int64_t, int, int, int,
                                  const DomainRec*, const float4*, double, double,
-                                 const MapProgram&, const Sched&, hipStream_t);
+                                 const MapProgram&, const Sched&, const double*, const uint8_t*,
+                                 hipStream_t);
 
 }  // namespace sc3d
