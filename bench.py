@@ -216,9 +216,12 @@ def main():
 
 def timed_sc_run(args, sc, S, config, steps, warmup):
     """Warm up, then time `steps` split-operator steps of a named workload:
-    the space-charge applies by HIP events per phase on the apply stream, the
-    whole loop (maps included) by the wall clock.  Returns the record, and
-    the bunch, options and propagator for the CPU leg."""
+    the space-charge applies by one HIP event pair per apply on the apply
+    stream (the value), the whole loop (maps included) by the wall clock;
+    then the same steps of a fresh copy of the workload with events at every
+    phase boundary (the per-phase split and the per-kernel rooflines; those
+    events add a few us each).  Returns the record, and the bunch, options
+    and propagator for the CPU leg."""
     import torch
     cfg = CONFIGS[config]
     n, grid, precision = cfg["n"], cfg["grid"], cfg["precision"]
@@ -227,8 +230,8 @@ def timed_sc_run(args, sc, S, config, steps, warmup):
           f"sort_period={opts.sort_period}", file=sys.stderr, flush=True)
     prop.run(warmup, sync=True)
     torch.cuda.synchronize()
-    op.set_timing(True)
-    op.timing(reset=True)
+    op.set_timing(sc.TIMING_APPLY)
+    op.apply_timing(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prop.run(steps, fuse=False)
@@ -236,13 +239,29 @@ def timed_sc_run(args, sc, S, config, steps, warmup):
     t1 = time.perf_counter()
     op.sync()
     wall = t1 - t0
-    phases, napplies = op.timing(reset=True)
+    apply_ms, napplies = op.apply_timing(reset=True)
     assert napplies == steps
-    sc_s = sum(phases.values()) / 1e3
+    op.set_timing(sc.TIMING_OFF)
+    del op, prop, bunch
+    torch.cuda.empty_cache()
+    sc_s = apply_ms / 1e3
     ms_per_step = 1e3 * sc_s / steps
     value = n * steps / sc_s
     print(f"# SC steps {sc_s:.4f}s -> {ms_per_step:.4f} ms/step, {value:.4e} particle-kicks/s; "
           f"tracking wall {wall:.4f}s incl. maps -> {n * steps / wall:.4e} /s",
+          file=sys.stderr, flush=True)
+    # the same steps again, phases timed
+    bunch, opts, op, prop = make_workload(sc, S, config, sort_period=args.sort_period)
+    prop.run(warmup, sync=True)
+    torch.cuda.synchronize()
+    op.set_timing(sc.TIMING_PHASES)
+    op.timing(reset=True)
+    prop.run(steps, fuse=False)
+    op.sync()
+    phases, napplies = op.timing(reset=True)
+    assert napplies == steps
+    phase_ms = sum(phases.values()) / steps
+    print(f"# phases (events at every boundary, {phase_ms:.4f} ms/step in all):",
           file=sys.stderr, flush=True)
     for k, v in phases.items():
         print(f"#   {k:16s} {v / steps * 1e3:9.2f} us/step", file=sys.stderr)
@@ -275,12 +294,13 @@ def timed_sc_run(args, sc, S, config, steps, warmup):
             "particles": n, "grid": list(grid), "doubled_grid": [2 * x for x in grid],
             "green_fn": "linear", "domain": "adaptive", "sort_period": opts.sort_period,
             "phase_us_per_step": {k: round(v / steps * 1e3, 2) for k, v in phases.items()},
+            "phase_sum_ms_per_step": round(phase_ms, 4),
             "tracking_ms_per_step_incl_maps": round(1e3 * wall / steps, 4),
             "tracking_particle_steps_per_s": n * steps / wall,
         },
         "roofline": roofline,
     }
-    op.set_timing(False)
+    op.set_timing(sc.TIMING_OFF)
     # the same number of steps again with the maps after each kick fused into
     # the kick kernel (one particle pass for kick + maps, as the served form
     # runs them): wall time of the tracking loop

@@ -147,7 +147,16 @@ int sc3d_debug_get(sc3d_ctx* ctx, int what, void* host_dst, int64_t bytes);
  *   SC3D_PH_FIELD   get_force            (E field kernel)
  *   SC3D_PH_KICK    sc3d_kick            (the kick kernel only)
  * sc3d_get_timing synchronises and returns milliseconds summed over the
- * applies timed since the last reset, and their count. */
+ * applies timed since the last reset, and their count.
+ * sc3d_set_timing modes: SC3D_TIMING_OFF; SC3D_TIMING_PHASES (events at
+ * every phase boundary, read by sc3d_get_timing); SC3D_TIMING_APPLY (one
+ * event pair around each whole apply, read by sc3d_get_apply_timing: the
+ * phase events add a few microseconds of device time each, so the whole
+ * apply is timed best without them).  No reference counterpart: the
+ * reference times with simple_timer on the host. */
+#define SC3D_TIMING_OFF 0
+#define SC3D_TIMING_PHASES 1
+#define SC3D_TIMING_APPLY 2
 #define SC3D_PH_DOMAIN 0
 #define SC3D_PH_SORT 1
 #define SC3D_PH_DEPOSIT 2
@@ -158,6 +167,7 @@ int sc3d_debug_get(sc3d_ctx* ctx, int what, void* host_dst, int64_t bytes);
 #define SC3D_NPHASES 7
 int sc3d_set_timing(sc3d_ctx* ctx, int enable);
 int sc3d_get_timing(sc3d_ctx* ctx, double ms[SC3D_NPHASES], int64_t* napplies, int reset);
+int sc3d_get_apply_timing(sc3d_ctx* ctx, double* ms, int64_t* napplies, int reset);
 
 /* Domain record copied by SC3D_DBG_DOMAIN (all FP64). */
 typedef struct sc3d_domain_record {

@@ -120,10 +120,12 @@ MAP_QUADRUPOLE = 1
 MAX_SLICES = 16
 PHASES = ("sc3d_domain", "sc3d_sort", "sc3d_local_rho", "sc3d_green_fn2", "sc3d_local_f",
           "sc3d_force", "sc3d_kick")
+TIMING_OFF, TIMING_PHASES, TIMING_APPLY = 0, 1, 2
 
 EXPORTS = ("sc3d_options_default", "sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy",
            "sc3d_last_error", "sc3d_debug_get", "sc3d_populate_gaussian", "sc3d_reference_cdt",
-           "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_set_post_kick_maps",
+           "sc3d_apply_maps", "sc3d_set_timing", "sc3d_get_timing", "sc3d_get_apply_timing",
+           "sc3d_set_post_kick_maps",
            "sc3d_moments", "sc3d_tail_fraction", "sc3d_green_patches",
            "sc3d_moments_noalloc", "sc3d_adjust_moments",
            # served form (include/sc3d_serve.h)
@@ -164,6 +166,8 @@ def load(path=None):
                                C.POINTER(C.c_double), C.c_void_p]
     L.sc3d_get_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.c_int]
+    L.sc3d_get_apply_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int64), C.c_int]
     L.sc3d_tail_fraction.argtypes = [C.c_void_p, C.POINTER(BunchDesc), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.c_double,
                                      C.POINTER(C.c_double), C.c_void_p]
@@ -178,7 +182,8 @@ def load(path=None):
     L.sc3d_adjust_moments.restype = C.c_int
     for name in ("sc3d_create", "sc3d_apply", "sc3d_sync", "sc3d_destroy", "sc3d_debug_get",
                  "sc3d_populate_gaussian", "sc3d_apply_maps", "sc3d_set_timing",
-                 "sc3d_get_timing", "sc3d_set_post_kick_maps", "sc3d_moments",
+                 "sc3d_get_timing", "sc3d_get_apply_timing", "sc3d_set_post_kick_maps",
+                 "sc3d_moments",
                  "sc3d_tail_fraction"):
         getattr(L, name).restype = C.c_int
     if path is None:
@@ -380,8 +385,10 @@ class SpaceCharge3dOpenHockney:
                                          _stream_ptr(stream)))
         return out
 
-    def set_timing(self, enable=True):
-        _check(load().sc3d_set_timing(self._ctx, int(enable)))
+    def set_timing(self, mode=TIMING_PHASES):
+        """TIMING_OFF, TIMING_PHASES (timing()) or TIMING_APPLY (apply_timing());
+        True / False for phases / off."""
+        _check(load().sc3d_set_timing(self._ctx, int(mode)))
 
     def timing(self, reset=False):
         """(dict phase -> total ms, number of timed applies)."""
@@ -389,6 +396,13 @@ class SpaceCharge3dOpenHockney:
         n = C.c_int64()
         _check(load().sc3d_get_timing(self._ctx, ms, C.byref(n), int(reset)))
         return dict(zip(PHASES, list(ms))), n.value
+
+    def apply_timing(self, reset=False):
+        """(total ms of the whole applies timed, their number)."""
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(load().sc3d_get_apply_timing(self._ctx, C.byref(ms), C.byref(n), int(reset)))
+        return ms.value, n.value
 
     def close(self):
         if self._ctx:

@@ -147,9 +147,10 @@ void dfree(sc3d_ctx* c, void* p)
         }
 }
 
-hipEvent_t mark(sc3d_ctx* c, hipStream_t st)
+hipEvent_t mark(sc3d_ctx* c, hipStream_t st, bool edge)
 {
-    if (!c->timing) return nullptr;
+    if (c->timing != SC3D_TIMING_PHASES && !(c->timing == SC3D_TIMING_APPLY && edge))
+        return nullptr;
     if (c->ev_used == c->ev_pool.size()) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
@@ -249,7 +250,7 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
         n > 0 && local_dep &&
         (sc.ctl != nullptr ? sc.sort_now != 0 : (c->applies % o.sort_period) == 0);
     const bool do_sort = n > 0 && (do_reorder || !local_dep);
-    ev[0] = mark(c, st);
+    ev[0] = mark(c, st, true);
     launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, sc, c->rho, c->rho_bytes,
                   st);
     ev[1] = mark(c, st);
@@ -477,7 +478,7 @@ int sc3d_apply(sc3d_ctx* c, const sc3d_bunch* b, double time_step, void* stream)
     enqueue_front(c, b, time_step, none, st, ev);
     enqueue_grid(&c, 1, c->d_gj, st, ev);
     enqueue_kick(c, b, none, st);
-    ev[7] = mark(c, st);
+    ev[7] = mark(c, st, true);
     (void)ev;
     HIP_TRY(hipGetLastError());
     c->applies++;
@@ -601,10 +602,14 @@ int sc3d_set_post_kick_maps(sc3d_ctx* c, const sc3d_map_slice* slices, int nslic
     return SC3D_OK;
 }
 
-int sc3d_set_timing(sc3d_ctx* c, int enable)
+int sc3d_set_timing(sc3d_ctx* c, int mode)
 {
     if (!c) return fail(SC3D_ERR_ARG, "sc3d_set_timing: null context");
-    c->timing = enable != 0;
+    if (mode < SC3D_TIMING_OFF || mode > SC3D_TIMING_APPLY)
+        return fail(SC3D_ERR_ARG, "sc3d_set_timing: unknown mode");
+    if (mode != c->timing && c->ev_used > 0)
+        return fail(SC3D_ERR_ARG, "sc3d_set_timing: read the timing before changing the mode");
+    c->timing = mode;
     return SC3D_OK;
 }
 
@@ -613,6 +618,8 @@ int sc3d_get_timing(sc3d_ctx* c, double ms[SC3D_NPHASES], int64_t* napplies, int
     if (!c || !ms) return fail(SC3D_ERR_ARG, "sc3d_get_timing: null argument");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->last_stream));
+    if (c->timing == SC3D_TIMING_APPLY)
+        return fail(SC3D_ERR_ARG, "sc3d_get_timing: whole applies timed (sc3d_get_apply_timing)");
     const size_t per = SC3D_NPHASES + 1;
     for (size_t k = 0; k + per <= c->ev_used; k += per) {
         for (int p = 0; p < SC3D_NPHASES; ++p) {
@@ -628,6 +635,29 @@ int sc3d_get_timing(sc3d_ctx* c, double ms[SC3D_NPHASES], int64_t* napplies, int
     if (reset) {
         for (int p = 0; p < SC3D_NPHASES; ++p) c->ph_ms[p] = 0.0;
         c->ph_applies = 0;
+    }
+    return SC3D_OK;
+}
+
+int sc3d_get_apply_timing(sc3d_ctx* c, double* ms, int64_t* napplies, int reset)
+{
+    if (!c || !ms) return fail(SC3D_ERR_ARG, "sc3d_get_apply_timing: null argument");
+    if (c->timing == SC3D_TIMING_PHASES)
+        return fail(SC3D_ERR_ARG, "sc3d_get_apply_timing: phases timed (sc3d_get_timing)");
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->ev_used > 0) HIP_TRY(hipStreamSynchronize(c->last_stream));
+    for (size_t k = 0; k + 2 <= c->ev_used; k += 2) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, c->ev_pool[k], c->ev_pool[k + 1]));
+        c->apply_ms += t;
+        c->apply_n++;
+    }
+    c->ev_used = 0;
+    *ms = c->apply_ms;
+    if (napplies) *napplies = c->apply_n;
+    if (reset) {
+        c->apply_ms = 0.0;
+        c->apply_n = 0;
     }
     return SC3D_OK;
 }

@@ -41,11 +41,14 @@ struct sc3d_ctx {
     int64_t sorted_n = -1;
     hipStream_t last_stream = nullptr;
     std::vector<void*> owned;
-    // phase timing: one set of SC3D_NPHASES+1 events per timed apply
-    bool timing = false;
+    // timing (SC3D_TIMING_*): SC3D_NPHASES+1 events per timed apply, or 2
+    // (whole applies)
+    int timing = SC3D_TIMING_OFF;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     double ph_ms[SC3D_NPHASES] = {0};
+    double apply_ms = 0.0;
+    int64_t apply_n = 0;
     int64_t ph_applies = 0;
     sc3d::MapProgram post{};  // maps fused into the kick kernel (post.n == 0: none)
     double* M = nullptr;      // moment partials (diagnostics)
@@ -94,8 +97,10 @@ int dmalloc(sc3d_ctx* c, T** p, int64_t n)
 
 void dfree(sc3d_ctx* c, void* p);
 
-// Records the next pooled timing event on st (timing on), else nullptr.
-hipEvent_t mark(sc3d_ctx* c, hipStream_t st);
+// Records the next pooled timing event on st when timing phases, or when
+// timing whole applies and `edge` (the apply's first and last mark); else
+// nullptr.
+hipEvent_t mark(sc3d_ctx* c, hipStream_t st, bool edge = false);
 
 // Device bytes a context of these options holds once it has applied to a
 // bunch of n particles (the admission footprint of the served form).
