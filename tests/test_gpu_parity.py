@@ -336,6 +336,39 @@ def test_kick_field_tile_and_direct_gathers(gpu, precision):
     check_kicked(pg, mg, out["parts"], p1, tol=tol, dk_tol=dk_tol)
 
 
+def test_timing_modes(gpu):
+    """Phase timing and whole-apply timing: each read back by its own entry
+    point (the other refuses), a mode change refused while events are
+    unread, and every timed apply counted."""
+    n = 20000
+    shape = (16, 16, 32)
+    parts = gaussian_parts(n, C1_SIGMA, seed=61)
+    masks = np.ones(n, np.uint8)
+    b = sc.Bunch(n, 5e10, gamma=GAMMA)
+    b.set_host(parts, masks)
+    op = sc.SpaceCharge3dOpenHockney(make_opts(shape))
+    op.set_timing(sc.TIMING_APPLY)
+    for _ in range(3):
+        op.apply_bunch(b, 1e-9)
+    with pytest.raises(sc.Sc3dError):
+        op.set_timing(sc.TIMING_PHASES)  # three applies' events unread
+    with pytest.raises(sc.Sc3dError):
+        op.timing()
+    ms, napplies = op.apply_timing(reset=True)
+    assert napplies == 3 and ms > 0.0
+    op.set_timing(sc.TIMING_PHASES)
+    for _ in range(2):
+        op.apply_bunch(b, 1e-9)
+    with pytest.raises(sc.Sc3dError):
+        op.apply_timing()
+    phases, napplies = op.timing(reset=True)
+    assert napplies == 2 and all(v >= 0.0 for v in phases.values())
+    assert phases["sc3d_kick"] > 0.0
+    op.set_timing(sc.TIMING_OFF)
+    op.apply_bunch(b, 1e-9)
+    assert op.timing()[1] == 0
+
+
 def test_multi_step_parity(gpu):
     """Three kicks in a row (sort every 2nd apply) vs three oracle kicks."""
     n = 30000
