@@ -119,6 +119,10 @@ def main():
     ap.add_argument("--jobs", type=int, default=256, help="served: jobs in the stream")
     ap.add_argument("--branches", type=int, default=8,
                     help="served: streams the per-job chains are spread over")
+    ap.add_argument("--admit-interval", type=int, default=8,
+                    help="served: iterations between admissions (graph rebuilds)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="served: launch every iteration's kernels directly")
     ap.add_argument("--form", choices=("served", "plain64", "plain32"), default=None,
                     help="run only one form on the job stream, all jobs at once (served) or "
                          "one after another through the operator API (plain64 / plain32), and "
@@ -433,7 +437,7 @@ def served_record(args, sc, S):
           f"{stream[-1][0]:.2f} s", file=sys.stderr, flush=True)
 
     def drive(real_time):
-        srv = SV.Server(SV.options(branches=args.branches))
+        srv = SV.Server(server_options(SV, args))
         pending = list(stream)
         ids = []
         t0 = time.perf_counter()
@@ -497,6 +501,11 @@ def served_record(args, sc, S):
     }
 
 
+def server_options(SV, args):
+    return SV.options(branches=args.branches, admit_interval=args.admit_interval,
+                      use_graphs=not args.no_graphs)
+
+
 def form_run(args, sc, S):
     """One form alone on the first `--jobs` jobs of the stream (seed 4): the
     served form with every job submitted at t = 0, or the plain operator
@@ -510,7 +519,7 @@ def form_run(args, sc, S):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if args.form == "served":
-        srv = SV.Server(SV.options(branches=args.branches))
+        srv = SV.Server(server_options(SV, args))
         for _, kw in stream:
             srv.submit(SV.Job(**kw))
         while True:
