@@ -71,14 +71,16 @@ template <int M> struct XLines {
     static constexpr int value = M >= 128 ? Lines<M>::value : HalfLines<M>::value;
 };
 
-// Lines per block, from a preferred count, halved (down to 2) while the
-// block's LDS (K lines per transform line plus the twiddle table) exceeds
-// the cap: the passes are latency-bound at low occupancy, and gfx950 has
-// 160 KiB of LDS per CU.
-// (Measured at 64x64x128 and 128x128x256: 32 KiB for FP64 lines (24 KiB
-// the same, 48 KiB slower); 32 KiB for FP32 lines (field passes 273 us vs
-// 280 us at 48 KiB, 303 us at 64 KiB).  16 KiB for both: solve 57 -> 62 us
-// at 64x64x128, 265 -> 291 us at 128x128x256.)
+// Lines per block of the LDS passes that hold whole line blocks (the
+// precision form's x' pass, the Green's column passes), from a preferred
+// count, halved (down to 2) while the block's LDS (K lines per transform
+// line plus the twiddle table) exceeds the cap: the passes are
+// latency-bound at low occupancy, and gfx950 has 160 KiB of LDS per CU.
+// (Measured when every transform pass held its lines this way, at 64x64x128
+// and 128x128x256: 32 KiB for FP64 lines (24 KiB the same, 48 KiB slower);
+// 32 KiB for FP32 lines (field passes 273 us vs 280 us at 48 KiB, 303 us at
+// 64 KiB).  16 KiB for both: solve 57 -> 62 us at 64x64x128, 265 -> 291 us
+// at 128x128x256.)
 #ifndef SC3D_LDS_CAP_FP64
 #define SC3D_LDS_CAP_FP64 32768
 #endif
@@ -548,9 +550,10 @@ __device__ __forceinline__ void reg_fft(T2 (&v)[(RegPlan<L, T2>::E)], T2* lb, in
 }
 
 // Lines per block of the register-resident column passes: 128-byte runs
-// along the line direction (8 FP64 or 16 FP32 complex values), fewer where
-// the block's LDS would pass the cap, and at least one wavefront of
-// threads.
+// across the lines (8 FP64 or 16 FP32 complex values: neighbouring lines
+// are contiguous), fewer where the block's LDS (line buffers plus the
+// twiddle table) would pass the cap, and at least one wavefront of threads.
+// (Measured at 64x64x128 FP64, z pass: 128-byte runs 16.6 us, 64-byte 17.9.)
 #ifndef SC3D_REG_RUN_BYTES
 #define SC3D_REG_RUN_BYTES 128
 #endif
