@@ -1020,7 +1020,9 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // (deposit phase per step, 4e6 FP64 / 1.6e7 FP32): 4080 nodes (32 KiB with
 // the box, 5 blocks per CU) 56.5 / 224.7 against 55.7 / 215.5; the same at 5
 // waves per SIMD (96 VGPRs, 32 bytes spilled) 60.0 / 228.8; 3060 nodes at 6
-// waves 93.3 / 351.9; the last eighth rotated first 55.8 / 215.7.
+// waves 93.3 / 351.9; the last eighth rotated first 55.8 / 215.7; 2 or 8
+// slots per thread (windows of 512 / 2048) 61.6 / 242.4 and 72.6 / 287.0
+// against 54.8 / 215.3.
 constexpr int kTileThreads = 256;
 constexpr int kTilePer = 4;                             // consecutive slots per thread
 constexpr int kTileWindow = kTileThreads * kTilePer;    // slots per block
