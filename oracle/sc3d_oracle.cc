@@ -223,9 +223,22 @@ extern "C" int32_t orc_cell_index(double pos, double left, double inv_h,
     return idx;
 }
 
-// Build-side cell key: k = (iz*gy + iy)*gx + ix for active particles whose
-// lower corner lies in [0, g-2]^3 (exactly the particles that the deposit
-// and the kick touch), else G = gx*gy*gz.
+// Build-side cell key (the library's sort order, not the reference's: the
+// reference does not sort).  Cells in bricks of 4 per axis (1 on an axis
+// whose length 4 does not divide), bricks and the cells inside one brick
+// each in (z, y, x) order with x fastest:
+//   k = ((bz*nby + by)*nbx + bx) * (Bx*By*Bz) + (lz*By + ly)*Bx + lx
+// for active particles whose lower corner lies in [0, g-2]^3 (exactly the
+// particles that the deposit and the kick touch), else G = gx*gy*gz.
+static int32_t orc_brick_key(int32_t ix, int32_t iy, int32_t iz, const int32_t g[3])
+{
+    const int32_t B[3] = {g[0] % 4 == 0 ? 4 : 1, g[1] % 4 == 0 ? 4 : 1, g[2] % 4 == 0 ? 4 : 1};
+    const int32_t nbx = g[0] / B[0], nby = g[1] / B[1];
+    const int32_t brick = ((iz / B[2]) * nby + iy / B[1]) * nbx + ix / B[0];
+    const int32_t cell = ((iz % B[2]) * B[1] + iy % B[1]) * B[0] + ix % B[0];
+    return brick * (B[0] * B[1] * B[2]) + cell;
+}
+
 extern "C" void orc_cell_keys(const orc_bunch* b, const orc_domain* d,
                               const int32_t g[3], int32_t* keys)
 {
@@ -240,7 +253,7 @@ extern "C" void orc_cell_keys(const orc_bunch* b, const orc_domain* d,
                                     ih[2], &o);
         bool in = b->masks[i] && ix >= 0 && ix < g[0] - 1 && iy >= 0 &&
                   iy < g[1] - 1 && iz >= 0 && iz < g[2] - 1;
-        keys[i] = in ? (iz * g[1] + iy) * g[0] + ix : G;
+        keys[i] = in ? orc_brick_key(ix, iy, iz, g) : G;
     }
 }
 

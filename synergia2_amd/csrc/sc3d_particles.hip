@@ -361,9 +361,33 @@ __global__ __launch_bounds__(kThreads) void k_domain_partial(
 }
 
 // ---------------------------------------------------------------------------
-// Cell keys: k = (iz*gy + iy)*gx + ix for active particles with lower corner
-// in [0, g-2]^3 (the particles the deposit and the kick touch), else G.
+// Cell keys in brick order: the cells are grouped in bricks of 4 cells per
+// axis (an axis whose length 4 does not divide keeps bricks of 1 there);
+// bricks in (z, y, x) order, x fastest, and the cells of one brick likewise:
+//   k = ((bz*nby + by)*nbx + bx) * |brick| + (lz*By + ly)*Bx + lx,
+// a bijection of the cells onto [0, G).  Active particles with lower corner
+// in [0, g-2]^3 (the particles the deposit and the kick touch) get their
+// cell's key, the rest G.  A window of consecutive particles of this order
+// spans a compact box of cells: the deposit's and the kick's node tiles, and
+// the reorder's reads, stay local.
 // ---------------------------------------------------------------------------
+constexpr int kBrickLog2 = 2;  // 4 cells per axis
+
+// log2 of the brick length along an axis of g cells
+__device__ __forceinline__ int brick_log2(int g)
+{
+    return (g & ((1 << kBrickLog2) - 1)) == 0 ? kBrickLog2 : 0;
+}
+
+__device__ __forceinline__ int32_t brick_key(int ix, int iy, int iz, int gx, int gy, int gz)
+{
+    const int sx = brick_log2(gx), sy = brick_log2(gy), sz = brick_log2(gz);
+    const int brick = ((iz >> sz) * (gy >> sy) + (iy >> sy)) * (gx >> sx) + (ix >> sx);
+    const int cell = (((iz & ((1 << sz) - 1)) << sy) + (iy & ((1 << sy) - 1))) << sx |
+                     (ix & ((1 << sx) - 1));
+    return (brick << (sx + sy + sz)) + cell;
+}
+
 __device__ __forceinline__ int32_t cell_key(const double* __restrict__ parts, int64_t stride,
                                             const uint8_t* __restrict__ masks, int64_t i, int gx,
                                             int gy, int gz, const DomainRec& d)
@@ -374,7 +398,7 @@ __device__ __forceinline__ int32_t cell_key(const double* __restrict__ parts, in
     const int iz = cell_index(parts[i + 4 * stride], d.left[2], d.inv_h[2], o);
     const bool in = masks[i] && ix >= 0 && ix < gx - 1 && iy >= 0 && iy < gy - 1 && iz >= 0 &&
                     iz < gz - 1;
-    return in ? (iz * gy + iy) * gx + ix : gx * gy * gz;
+    return in ? brick_key(ix, iy, iz, gx, gy, gz) : gx * gy * gz;
 }
 
 // ---------------------------------------------------------------------------

@@ -58,3 +58,52 @@ def test_keys_and_stable_sort():
     O.lib().orc_stable_sort(k.ctypes.data_as(C.POINTER(C.c_int32)), 10000,
                             perm.ctypes.data_as(C.POINTER(C.c_int32)))
     assert np.array_equal(perm, np.argsort(k, kind="stable"))
+
+
+def _keys(g, cells, h=0.25, left=-0.5):
+    """orc_cell_keys of one particle at the centre of each cell (ix, iy, iz)."""
+    import ctypes as C
+    n = len(cells)
+    parts = np.zeros((7, n))
+    for i, c in enumerate(cells):
+        for a in range(3):
+            parts[2 * a, i] = left + (c[a] + 1.0) * h  # s = c + 1/2
+    masks = np.ones(n, np.uint8)
+    b = O.make_bunch(parts, masks, n, 1e10, 1, 0.938272046, GAMMA, BETA, PREF)
+    dom = O.Domain()
+    for a in range(3):
+        dom.left[a] = left
+        dom.h[a] = h
+    keys = np.zeros(n, np.int32)
+    O.lib().orc_cell_keys(C.byref(b), C.byref(dom), np.array(g, np.int32).ctypes.data_as(
+        C.POINTER(C.c_int32)), keys.ctypes.data_as(C.POINTER(C.c_int32)))
+    return keys
+
+
+def test_brick_keys():
+    """Brick order: 4 cells per axis where 4 divides the axis (x and z of
+    8 x 6 x 12), 1 elsewhere; keys worked out by hand, then the bijection
+    onto [0, G) and the bricks as contiguous key ranges."""
+    g = (8, 6, 12)
+    # (ix, iy, iz) -> brick ((bz*6 + by)*2 + bx), cell ((lz*1 + 0)*4 + lx), key brick*16 + cell
+    hand = {(0, 0, 0): 0, (5, 2, 7): 17 * 16 + 13, (3, 4, 0): 8 * 16 + 3,
+            (4, 0, 1): 1 * 16 + 4, (6, 4, 10): ((2 * 6 + 4) * 2 + 1) * 16 + 2 * 4 + 2}
+    cells = list(hand)
+    assert list(_keys(g, cells)) == [hand[c] for c in cells]
+    G = g[0] * g[1] * g[2]
+    # every cell whose lower corner is in [0, g-2]^3 has a distinct key < G;
+    # the last layer of each axis gets G
+    cells = [(x, y, z) for z in range(g[2]) for y in range(g[1]) for x in range(g[0])]
+    k = _keys(g, cells)
+    inner = np.array([all(c[a] <= g[a] - 2 for a in range(3)) for c in cells])
+    assert np.all(k[~inner] == G)
+    assert len(set(k[inner].tolist())) == inner.sum() and k[inner].max() < G
+    # the 4 x 1 x 4 cells of one brick hold 16 consecutive keys
+    for bz in range(g[2] // 4 - 1):
+        for by in range(g[1] - 1):
+            kb = _keys(g, [(x, by, 4 * bz + z) for z in range(4) for x in range(4)])
+            assert sorted(kb.tolist()) == list(range(kb.min(), kb.min() + 16))
+    # an axis that 4 does not divide keeps the (z, y, x) order of the cells
+    g = (5, 6, 7)
+    cells = [(x, y, z) for z in range(4) for y in range(5) for x in range(4)]
+    assert list(_keys(g, cells)) == [(z * 6 + y) * 5 + x for x, y, z in cells]
