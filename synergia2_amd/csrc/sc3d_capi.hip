@@ -215,8 +215,20 @@ int prepare_apply(sc3d_ctx* c, const sc3d_bunch* b)
     return SC3D_OK;
 }
 
+// Green's spectrum of nj contexts of one shape (and the spectral patch
+// correction of the contexts that have one), on st.
+static void enqueue_green(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream_t st)
+{
+    sc3d_ctx* c = ctxs[0];
+    if (!c->fp32)
+        launch_green<double>(c->g, c->opt.green_fn, table, nj, c->tw64, st);
+    else
+        launch_green<float>(c->g, c->opt.green_fn, table, nj, c->tw32, st);
+    for (int j = 0; j < nj; ++j) launch_green_chat(ctxs[j], st);
+}
+
 int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sched& sc,
-                  hipStream_t st, hipEvent_t* ev)
+                  hipStream_t st, hipEvent_t* ev, hipStream_t green_st)
 {
     const int64_t n = b->n_active;
     const Geometry& g = c->g;
@@ -254,6 +266,12 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
     launch_domain(b->parts, b->stride, b->masks, n, c->P, a, c->dom, sc, c->rho, c->rho_bytes,
                   st);
     ev[1] = mark(c, st);
+    if (green_st) {
+        (void)hipEventRecord(c->fork_ev, st);
+        (void)hipStreamWaitEvent(green_st, c->fork_ev, 0);
+        enqueue_green(&c, 1, c->d_gj, green_st);
+        (void)hipEventRecord(c->join_ev, green_st);
+    }
     const int32_t* dep_perm = nullptr;
     if (do_sort) {
         launch_sort(b->parts, b->stride, b->masks, n, g.gx, g.gy, g.gz, c->dom, c->sb, st);
@@ -282,14 +300,16 @@ int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sche
 }
 
 void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream_t st,
-                  hipEvent_t* ev)
+                  hipEvent_t* ev, bool green_forked)
 {
     sc3d_ctx* c = ctxs[0];
     const Geometry& g = c->g;
+    if (green_forked)
+        (void)hipStreamWaitEvent(st, c->join_ev, 0);
+    else
+        enqueue_green(ctxs, nj, table, st);
+    ev[4] = mark(c, st);
     if (!c->fp32) {
-        launch_green<double>(g, c->opt.green_fn, table, nj, c->tw64, st);
-        for (int j = 0; j < nj; ++j) launch_green_chat(ctxs[j], st);
-        ev[4] = mark(c, st);
         launch_solve<double>(g, table, nj, c->tw64, st);
         ev[5] = mark(c, st);
         for (int j = 0; j < nj; ++j) {
@@ -300,9 +320,6 @@ void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream
         }
         ev[6] = mark(c, st);
     } else {
-        launch_green<float>(g, c->opt.green_fn, table, nj, c->tw32, st);
-        for (int j = 0; j < nj; ++j) launch_green_chat(ctxs[j], st);
-        ev[4] = mark(c, st);
         // the field comes straight out of the spectrum (no potential, no
         // separate difference pass): sc3d_force stays empty in this form
         launch_solve_field<float>(g, table, nj, c->tw32, st);
@@ -462,6 +479,12 @@ int sc3d_destroy(sc3d_ctx* c)
 {
     if (!c) return SC3D_OK;
     if (c->last_stream || c->applies) (void)hipStreamSynchronize(c->last_stream);
+    if (c->green_st) {
+        (void)hipStreamSynchronize(c->green_st);
+        (void)hipStreamDestroy(c->green_st);
+    }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void* q : c->owned)
         if (q) (void)hipFree(q);
@@ -475,8 +498,16 @@ int sc3d_apply(sc3d_ctx* c, const sc3d_bunch* b, double time_step, void* stream)
     hipStream_t st = static_cast<hipStream_t>(stream);
     const Sched none{};
     hipEvent_t ev[SC3D_NPHASES + 1];
-    enqueue_front(c, b, time_step, none, st, ev);
-    enqueue_grid(&c, 1, c->d_gj, st, ev);
+    // the Green's spectrum beside the sort and the deposit, except when
+    // timing phases (whose events then bracket each phase alone)
+    const bool fork = c->timing != SC3D_TIMING_PHASES;
+    if (fork && !c->green_st) {
+        if (!c->fork_ev) HIP_TRY(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+        if (!c->join_ev) HIP_TRY(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+        HIP_TRY(hipStreamCreateWithFlags(&c->green_st, hipStreamNonBlocking));
+    }
+    enqueue_front(c, b, time_step, none, st, ev, fork ? c->green_st : nullptr);
+    enqueue_grid(&c, 1, c->d_gj, st, ev, fork);
     enqueue_kick(c, b, none, st);
     ev[7] = mark(c, st, true);
     (void)ev;

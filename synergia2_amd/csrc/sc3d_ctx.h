@@ -37,6 +37,12 @@ struct sc3d_ctx {
     // the apply being enqueued reorders the bunch: its deposit reads the
     // gathered copy in scratch / smask, and its kick writes it back
     bool reordering = false;
+    // Green's spectrum stream of the plain form (sc3d_apply): the spectrum
+    // depends only on the domain, so it runs beside the sort and the deposit
+    // (fork after the domain, join before the convolution); created on the
+    // first apply, default priority
+    hipStream_t green_st = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int64_t applies = 0;
     int64_t sorted_n = -1;
     hipStream_t last_stream = nullptr;
@@ -112,14 +118,19 @@ int prepare_apply(sc3d_ctx* c, const sc3d_bunch* b);
 // The particle front of an apply: domain, cell sort, reorder, density
 // (events ev[0..3] when timing).  With sc.ctl (served form) the step
 // control decides reorder, schedule and kick time step on the device.
+// With green_st, the context's Green's spectrum is enqueued there right
+// after the domain (fork on c->fork_ev, done at c->join_ev), beside the
+// sort and the deposit on st.
 int enqueue_front(sc3d_ctx* c, const sc3d_bunch* b, double time_step, const Sched& sc,
-                  hipStream_t st, hipEvent_t* ev);
+                  hipStream_t st, hipEvent_t* ev, hipStream_t green_st = nullptr);
 
 // The grid middle for nj contexts of one shape and precision: Green's
 // spectrum (where h changed), convolution and field, as batched launches
 // over the device table `table` (events ev[4..6] when timing).
+// green_forked: enqueue_front put the spectrum of the one context on its
+// side stream; st waits for it instead.
 void enqueue_grid(sc3d_ctx* const* ctxs, int nj, const GridJob* table, hipStream_t st,
-                  hipEvent_t* ev);
+                  hipEvent_t* ev, bool green_forked = false);
 
 // The points where the reference's g2 differs from the symmetric one at
 // this shape (sc3d_quirk.hip); returns their number.

@@ -369,6 +369,49 @@ def test_timing_modes(gpu):
     assert op.timing()[1] == 0
 
 
+@pytest.mark.parametrize("precision", [sc.FP64, sc.FP32_GRID])
+def test_green_side_stream(gpu, precision):
+    """The Green's spectrum on its own stream beside the sort and the deposit
+    (timing off) against the same applies with every pass on one stream
+    (phase timing), over applies whose domain widens by 5 % each time (the
+    spectrum recomputed each time; a spectrum of the previous domain would
+    change the kicks by percents) and sort periods of 2.  Positions, ids and
+    masks identical; the kicks agree to the rounding of the deposit's atomic
+    sums, whose order varies from run to run."""
+    n = 40000
+    shape = (16, 32, 64)
+    parts = gaussian_parts(n, C1_SIGMA, seed=71)
+    masks = np.ones(n, np.uint8)
+    masks[::29] = 0
+    out = []
+    for mode in (sc.TIMING_OFF, sc.TIMING_PHASES):
+        print(f"case green side stream precision={precision} mode={mode}", flush=True)
+        op = sc.SpaceCharge3dOpenHockney(make_opts(shape, sort_period=2, precision=precision))
+        op.set_timing(mode)
+        b = sc.Bunch(n, 5e10, gamma=GAMMA)
+        got = []
+        cur = parts.copy()
+        for step in range(3):
+            cur[0] *= 1.0 + 0.05 * step  # a wider domain each apply
+            b.set_host(cur, masks)
+            op.apply_bunch(b, 2e-9)
+            p, m = b.host()
+            got.append((cur.copy(), p.copy(), m.copy()))
+        if mode == sc.TIMING_PHASES:
+            assert op.timing(reset=True)[1] == 3
+        out.append(got)
+    dk_tol = 1e-10 if precision == sc.FP64 else 1e-5
+    for step, ((p0, pa, ma), (_, pb, mb)) in enumerate(zip(*out)):
+        assert np.array_equal(ma, mb)
+        for c in (0, 2, 4, 6):
+            assert np.array_equal(pa[c], pb[c]), (step, c)
+        a, z, s0 = by_id(pa), by_id(pb), p0[:, np.argsort(p0[6], kind="stable")]
+        for c in (1, 3, 5):
+            err = rel_l2(a[c] - s0[c], z[c] - s0[c])
+            print(f"  step {step} column {c}: kick rel-L2 {err:.2e}", flush=True)
+            assert err <= dk_tol, (step, c, err)
+
+
 def test_multi_step_parity(gpu):
     """Three kicks in a row (sort every 2nd apply) vs three oracle kicks."""
     n = 30000
