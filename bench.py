@@ -101,7 +101,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48,
-                    help="timed steps (a multiple of the default sort period, 12)")
+                    help="timed steps (a multiple of the default sort period, 24)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default=None, choices=sorted(CONFIGS),
                     help="time only this workload (default: the FP64 workload as the main line, "

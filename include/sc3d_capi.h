@@ -74,7 +74,7 @@ typedef struct sc3d_options {
                                     into that order (the deposit sorts windows
                                     of the storage on the device in between);
                                     0 = never reorder: sort on every apply and
-                                    deposit through the permutation; default 12*/
+                                    deposit through the permutation; default 24*/
 } sc3d_options;
 
 typedef struct sc3d_bunch {

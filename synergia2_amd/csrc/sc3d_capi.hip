@@ -374,7 +374,7 @@ void sc3d_options_default(sc3d_options* o, int32_t gx, int32_t gy, int32_t gz)
     o->domain_fixed = 0;
     o->comm_group_size = 4;
     o->precision = SC3D_PRECISION_FP64;
-    o->sort_period = 12;
+    o->sort_period = 24;
 }
 
 int sc3d_create(const sc3d_options* opt, sc3d_ctx** out)

@@ -55,7 +55,7 @@ struct Space_charge_3d_open_hockney_options {
     int comm_group_size;
     // device-side extensions
     int precision = 0;    // 0: FP64, 1: FP32 grid / FP64 particles
-    int sort_period = 12;  // reorder the bunch storage by cell every N applies
+    int sort_period = 24;  // reorder the bunch storage by cell every N applies
 
     Space_charge_3d_open_hockney_options(int gridx = 32, int gridy = 32, int gridz = 64)
         : shape{gridx, gridy, gridz}
