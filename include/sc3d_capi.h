@@ -124,9 +124,10 @@ const char* sc3d_last_error(void);
 #define SC3D_DBG_KEYS 1          /* int32[n] cell keys of the last sort, in  */
                                  /* the bunch order before that sort (brick */
                                  /* order: bricks of 4 cells per axis where */
-                                 /* 4 divides it, bricks and their cells    */
-                                 /* z-major, x fastest; G for particles     */
-                                 /* outside [0, g-2]^3 or masked out)       */
+                                 /* 4 divides it, on a serpentine path, x   */
+                                 /* fastest; cells of a brick z-major, x    */
+                                 /* fastest; G for particles outside        */
+                                 /* [0, g-2]^3 or masked out)               */
 #define SC3D_DBG_PERM 2          /* int32[n] stable sort permutation         */
 #define SC3D_DBG_RHO 3           /* double[G] charge density octant (zyx)    */
 #define SC3D_DBG_PHI 4           /* double[G] potential octant, unnormalised */

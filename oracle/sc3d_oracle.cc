@@ -225,16 +225,23 @@ extern "C" int32_t orc_cell_index(double pos, double left, double inv_h,
 
 // Build-side cell key (the library's sort order, not the reference's: the
 // reference does not sort).  Cells in bricks of 4 per axis (1 on an axis
-// whose length 4 does not divide), bricks and the cells inside one brick
-// each in (z, y, x) order with x fastest:
-//   k = ((bz*nby + by)*nbx + bx) * (Bx*By*Bz) + (lz*By + ly)*Bx + lx
+// whose length 4 does not divide).  Bricks on a serpentine path: row
+// r = bz*nby + by' with by' = by, reversed (nby-1-by) in odd layers bz;
+// bx' = bx, reversed (nbx-1-bx) in odd rows r.  Cells inside one brick in
+// (z, y, x) order with x fastest:
+//   k = (r*nbx + bx') * (Bx*By*Bz) + (lz*By + ly)*Bx + lx
 // for active particles whose lower corner lies in [0, g-2]^3 (exactly the
 // particles that the deposit and the kick touch), else G = gx*gy*gz.
 static int32_t orc_brick_key(int32_t ix, int32_t iy, int32_t iz, const int32_t g[3])
 {
     const int32_t B[3] = {g[0] % 4 == 0 ? 4 : 1, g[1] % 4 == 0 ? 4 : 1, g[2] % 4 == 0 ? 4 : 1};
     const int32_t nbx = g[0] / B[0], nby = g[1] / B[1];
-    const int32_t brick = ((iz / B[2]) * nby + iy / B[1]) * nbx + ix / B[0];
+    const int32_t bz = iz / B[2];
+    int32_t by = iy / B[1], bx = ix / B[0];
+    if (bz % 2 == 1) by = nby - 1 - by;
+    const int32_t r = bz * nby + by;
+    if (r % 2 == 1) bx = nbx - 1 - bx;
+    const int32_t brick = r * nbx + bx;
     const int32_t cell = ((iz % B[2]) * B[1] + iy % B[1]) * B[0] + ix % B[0];
     return brick * (B[0] * B[1] * B[2]) + cell;
 }

@@ -362,9 +362,13 @@ __global__ __launch_bounds__(kThreads) void k_domain_partial(
 
 // ---------------------------------------------------------------------------
 // Cell keys in brick order: the cells are grouped in bricks of 4 cells per
-// axis (an axis whose length 4 does not divide keeps bricks of 1 there);
-// bricks in (z, y, x) order, x fastest, and the cells of one brick likewise:
-//   k = ((bz*nby + by)*nbx + bx) * |brick| + (lz*By + ly)*Bx + lx,
+// axis (an axis whose length 4 does not divide keeps bricks of 1 there).
+// The bricks follow a serpentine path, x fastest: rows of bricks run
+// backwards in x on odd rows (counting rows r = bz*nby + by'), and the rows
+// of a layer run backwards in y on odd layers bz (by' = nby-1-by), so
+// consecutive bricks are always face neighbours; the cells of one brick are
+// in (z, y, x) order:
+//   k = (r*nbx + bx') * |brick| + (lz*By + ly)*Bx + lx,
 // a bijection of the cells onto [0, G).  Active particles with lower corner
 // in [0, g-2]^3 (the particles the deposit and the kick touch) get their
 // cell's key, the rest G.  A window of consecutive particles of this order
@@ -382,7 +386,12 @@ __device__ __forceinline__ int brick_log2(int g)
 __device__ __forceinline__ int32_t brick_key(int ix, int iy, int iz, int gx, int gy, int gz)
 {
     const int sx = brick_log2(gx), sy = brick_log2(gy), sz = brick_log2(gz);
-    const int brick = ((iz >> sz) * (gy >> sy) + (iy >> sy)) * (gx >> sx) + (ix >> sx);
+    const int nbx = gx >> sx, nby = gy >> sy, bz = iz >> sz;
+    int by = iy >> sy, bx = ix >> sx;
+    if (bz & 1) by = nby - 1 - by;  // layers of odd bz: rows backwards
+    const int row = bz * nby + by;
+    if (row & 1) bx = nbx - 1 - bx;  // odd rows: bricks backwards
+    const int brick = row * nbx + bx;
     const int cell = (((iz & ((1 << sz) - 1)) << sy) + (iy & ((1 << sy) - 1))) << sx |
                      (ix & ((1 << sx) - 1));
     return (brick << (sx + sy + sz)) + cell;

@@ -82,12 +82,18 @@ def _keys(g, cells, h=0.25, left=-0.5):
 
 def test_brick_keys():
     """Brick order: 4 cells per axis where 4 divides the axis (x and z of
-    8 x 6 x 12), 1 elsewhere; keys worked out by hand, then the bijection
-    onto [0, G) and the bricks as contiguous key ranges."""
+    8 x 6 x 12), 1 elsewhere, bricks on a serpentine path; keys worked out
+    by hand, then the bijection onto [0, G), the bricks as contiguous key
+    ranges and consecutive bricks as face neighbours."""
     g = (8, 6, 12)
-    # (ix, iy, iz) -> brick ((bz*6 + by)*2 + bx), cell ((lz*1 + 0)*4 + lx), key brick*16 + cell
-    hand = {(0, 0, 0): 0, (5, 2, 7): 17 * 16 + 13, (3, 4, 0): 8 * 16 + 3,
-            (4, 0, 1): 1 * 16 + 4, (6, 4, 10): ((2 * 6 + 4) * 2 + 1) * 16 + 2 * 4 + 2}
+    # (ix, iy, iz) -> brick (r*2 + bx'), r = bz*6 + by' (by' = 5 - by for odd bz,
+    # bx' = 1 - bx for odd r), cell (lz*4 + lx); key brick*16 + cell
+    hand = {(0, 0, 0): 0,
+            (5, 2, 7): (9 * 2 + 0) * 16 + 3 * 4 + 1,     # bz 1: by' 3, r 9 odd: bx' 0
+            (3, 4, 0): (4 * 2 + 0) * 16 + 3,             # r 4
+            (4, 0, 1): (0 * 2 + 1) * 16 + 1 * 4,         # r 0, bx 1, lz 1
+            (1, 1, 2): (1 * 2 + 1) * 16 + 2 * 4 + 1,     # r 1 odd: bx' 1
+            (6, 4, 10): (16 * 2 + 1) * 16 + 2 * 4 + 2}   # bz 2: r 16
     cells = list(hand)
     assert list(_keys(g, cells)) == [hand[c] for c in cells]
     G = g[0] * g[1] * g[2]
@@ -103,7 +109,16 @@ def test_brick_keys():
         for by in range(g[1] - 1):
             kb = _keys(g, [(x, by, 4 * bz + z) for z in range(4) for x in range(4)])
             assert sorted(kb.tolist()) == list(range(kb.min(), kb.min() + 16))
-    # an axis that 4 does not divide keeps the (z, y, x) order of the cells
-    g = (5, 6, 7)
-    cells = [(x, y, z) for z in range(4) for y in range(5) for x in range(4)]
-    assert list(_keys(g, cells)) == [(z * 6 + y) * 5 + x for x, y, z in cells]
+    # serpentine: the bricks of 8 x 8 x 12 in key order step to a face neighbour
+    g = (8, 8, 12)
+    bricks = [(bx, by, bz) for bz in range(3) for by in range(2) for bx in range(2)]
+    rank = _keys(g, [(4 * bx, 4 * by, 4 * bz) for bx, by, bz in bricks]) // 64
+    path = [bricks[i] for i in np.argsort(rank)]
+    assert sorted(rank.tolist()) == list(range(12))
+    for a, b in zip(path, path[1:]):
+        assert sum(abs(u - v) for u, v in zip(a, b)) == 1, (a, b)
+    # no axis that 4 divides: bricks of one cell, the serpentine path over
+    # the cells of 5 x 6 x 7 (r = z*6 + y', y' = 5 - y for odd z; x' = 4 - x
+    # for odd r; key r*5 + x')
+    hand = {(0, 0, 0): 0, (1, 1, 0): 1 * 5 + 3, (2, 3, 1): 8 * 5 + 2, (3, 4, 3): 19 * 5 + 1}
+    assert list(_keys((5, 6, 7), list(hand))) == list(hand.values())
