@@ -121,6 +121,8 @@ def main():
                     help="served: streams the per-job chains are spread over")
     ap.add_argument("--admit-interval", type=int, default=8,
                     help="served: iterations between admissions (graph rebuilds)")
+    ap.add_argument("--serve-sort-period", type=int, default=24,
+                    help="served: iterations between reorders of the resident bunches")
     ap.add_argument("--no-graphs", action="store_true",
                     help="served: launch every iteration's kernels directly")
     ap.add_argument("--form", choices=("served", "plain64", "plain32"), default=None,
@@ -503,7 +505,7 @@ def served_record(args, sc, S):
 
 def server_options(SV, args):
     return SV.options(branches=args.branches, admit_interval=args.admit_interval,
-                      use_graphs=not args.no_graphs)
+                      use_graphs=not args.no_graphs, sort_period=args.serve_sort_period)
 
 
 def form_run(args, sc, S):

@@ -385,7 +385,7 @@ void sc3d_server_options_default(sc3d_server_options* o)
 {
     std::memset(o, 0, sizeof(*o));
     o->precision = SC3D_PRECISION_FP32_GRID;
-    o->sort_period = 12;
+    o->sort_period = 24;
     o->use_graphs = 1;
     o->batch_grids = 1;
     o->admit_interval = 8;

@@ -152,7 +152,7 @@ class Job:
 
 def options(precision=sc.FP32_GRID, hbm_budget=0, max_resident=0, use_graphs=True,
             batch_grids=True, admit_interval=8, branches=8, keep_particles=False,
-            sort_period=12):
+            sort_period=24):
     o = ServerOptions()
     _bind().sc3d_server_options_default(C.byref(o))
     o.precision = precision
