@@ -39,10 +39,12 @@ struct sc3d_ctx {
     bool reordering = false;
     // Green's spectrum stream of the plain form (sc3d_apply): the spectrum
     // depends only on the domain, so it runs beside the sort and the deposit
-    // (fork after the domain, join before the convolution); created on the
-    // first apply, default priority
+    // (fork after the domain, join before the convolution) when the caller
+    // enqueues ahead of the device; created on first use, default priority.
+    // done_ev marks the end of the last apply (recorded once done_rec).
     hipStream_t green_st = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, done_ev = nullptr;
+    bool done_rec = false;
     int64_t applies = 0;
     int64_t sorted_n = -1;
     hipStream_t last_stream = nullptr;

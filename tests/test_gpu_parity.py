@@ -372,31 +372,38 @@ def test_timing_modes(gpu):
 @pytest.mark.parametrize("precision", [sc.FP64, sc.FP32_GRID])
 def test_green_side_stream(gpu, precision):
     """The Green's spectrum on its own stream beside the sort and the deposit
-    (timing off) against the same applies with every pass on one stream
-    (phase timing), over applies whose domain widens by 5 % each time (the
-    spectrum recomputed each time; a spectrum of the previous domain would
-    change the kicks by percents) and sort periods of 2.  Positions, ids and
-    masks identical; the kicks agree to the rounding of the deposit's atomic
-    sums, whose order varies from run to run."""
+    (forced by the SC3D_TEST_GREEN_FORK hook: the automatic choice forks only
+    when the caller enqueues ahead of the device) against the same applies
+    with every pass on one stream (phase timing), over applies whose domain
+    widens by 5 % each time (the spectrum recomputed each time; a spectrum
+    of the previous domain would change the kicks by percents) and sort
+    periods of 2.  Positions, ids and masks identical; the kicks agree to
+    the rounding of the deposit's atomic sums, whose order varies from run
+    to run."""
     n = 40000
     shape = (16, 32, 64)
     parts = gaussian_parts(n, C1_SIGMA, seed=71)
     masks = np.ones(n, np.uint8)
     masks[::29] = 0
     out = []
-    for mode in (sc.TIMING_OFF, sc.TIMING_PHASES):
+    import os
+    for mode, fork in ((sc.TIMING_OFF, "1"), (sc.TIMING_PHASES, "0")):
         print(f"case green side stream precision={precision} mode={mode}", flush=True)
-        op = sc.SpaceCharge3dOpenHockney(make_opts(shape, sort_period=2, precision=precision))
-        op.set_timing(mode)
-        b = sc.Bunch(n, 5e10, gamma=GAMMA)
-        got = []
-        cur = parts.copy()
-        for step in range(3):
-            cur[0] *= 1.0 + 0.05 * step  # a wider domain each apply
-            b.set_host(cur, masks)
-            op.apply_bunch(b, 2e-9)
-            p, m = b.host()
-            got.append((cur.copy(), p.copy(), m.copy()))
+        os.environ["SC3D_TEST_GREEN_FORK"] = fork
+        try:
+            op = sc.SpaceCharge3dOpenHockney(make_opts(shape, sort_period=2, precision=precision))
+            op.set_timing(mode)
+            b = sc.Bunch(n, 5e10, gamma=GAMMA)
+            got = []
+            cur = parts.copy()
+            for step in range(3):
+                cur[0] *= 1.0 + 0.05 * step  # a wider domain each apply
+                b.set_host(cur, masks)
+                op.apply_bunch(b, 2e-9)
+                p, m = b.host()
+                got.append((cur.copy(), p.copy(), m.copy()))
+        finally:
+            del os.environ["SC3D_TEST_GREEN_FORK"]
         if mode == sc.TIMING_PHASES:
             assert op.timing(reset=True)[1] == 3
         out.append(got)
