@@ -1036,7 +1036,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // its quarter (a crowded node of a scattered storage then takes one add per
 // wavefront), the rest going to rho directly.  Those wide
 // windows lie at the two ends of the cell order (the tails of the bunch), so
-// the block order is rotated to start the last sixteenth of the windows
+// the block order is rotated to start the last eighth of the windows
 // first and keep them off the end of the kernel.  The tile accumulates in
 // FP64 for both precisions (the precision form's unit weights are rounded
 // to FP32 once, at the global add).
@@ -1055,12 +1055,15 @@ __global__ __launch_bounds__(kThreads) void k_deposit(const double* __restrict__
 // waves per SIMD (96 VGPRs, 32 bytes spilled) 60.0 / 228.8; 3060 nodes at 6
 // waves 93.3 / 351.9; the last eighth rotated first 55.8 / 215.7; 2 or 8
 // slots per thread (windows of 512 / 2048) 61.6 / 242.4 and 72.6 / 287.0
-// against 54.8 / 215.3.
+// against 54.8 / 215.3.  With brick-ordered keys, sort period 24: the last
+// eighth / sixteenth / thirty-second rotated first, or none, 54.6-54.8 /
+// 55.4-55.5 / 54.3-54.5 / 56.1-56.3 at 4e6 and 200.6-200.7 / 201.3-202.4 /
+// 201.8-202.3 / 203.5 at 1.6e7; 3 slots per thread 57.4-57.6 / 212.2-212.5.
 constexpr int kTileThreads = 256;
 constexpr int kTilePer = 4;                             // consecutive slots per thread
 constexpr int kTileWindow = kTileThreads * kTilePer;    // slots per block
 constexpr int kTileNodes = 4096;                        // FP64 nodes of the LDS tile (32 KiB)
-constexpr int kTileTailDiv = 16;                        // 1/16 of the windows rotated first
+constexpr int kTileTailDiv = 8;                         // 1/8 of the windows rotated first
 constexpr int kTileWaves = kTileThreads / 64;
 
 __device__ __forceinline__ int wave_min(int v)
@@ -1180,7 +1183,7 @@ __global__ __launch_bounds__(kTileThreads) void k_deposit_tile(
     const double ihx = dom->inv_h[0], ihy = dom->inv_h[1], ihz = dom->inv_h[2];
     // precision form: unit weights (w0 and the unit-peak scale return at the kick)
     const double w0 = sizeof(R) == sizeof(double) ? dom->w0 : 1.0;
-    // the last sixteenth of the windows (one tail of the bunch) first
+    // the last eighth of the windows (one tail of the bunch) first
     const int nb = int(gridDim.x), tail = nb / kTileTailDiv;
     const int blk = (int(blockIdx.x) + nb - tail) % nb;
     const int64_t base = int64_t(blk) * kTileWindow + int64_t(t) * kTilePer;
