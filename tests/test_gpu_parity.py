@@ -419,6 +419,46 @@ def test_green_side_stream(gpu, precision):
             assert err <= dk_tol, (step, c, err)
 
 
+def test_green_fork_automatic(gpu):
+    """The automatic choice on the workload it is for: 2^20 + 4096 particles
+    on 64 x 64 x 64 (2^18 cells) tracked through FODO steps with the maps
+    between kicks and no wait between the steps, so the Green's spectrum
+    forks beside the particle front whenever the previous apply is still
+    running; against the same steps with the fork switched off
+    (SC3D_TEST_GREEN_FORK=0).  Ids and masks identical; every coordinate
+    within 1e-10 of its largest value (the runs differ by the order of the
+    deposit's atomic sums only)."""
+    import os
+    from synergia2_amd import stepper as S
+    n = (1 << 20) + 4096
+    shape = (64, 64, 64)
+    parts = gaussian_parts(n, C1_SIGMA, seed=72)
+    masks = np.ones(n, np.uint8)
+    out = []
+    for fork in (None, "0"):
+        print(f"case green fork automatic, hook {fork}", flush=True)
+        if fork is not None:
+            os.environ["SC3D_TEST_GREEN_FORK"] = fork
+        try:
+            op = sc.SpaceCharge3dOpenHockney(make_opts(shape, sort_period=24))
+            b = sc.Bunch(n, 5e10, gamma=GAMMA)
+            b.set_host(parts, masks)
+            S.Propagator(S.fodo_channel(2), 8, b, op).run(6, fuse=False)
+            op.sync()
+            p, m = b.host()
+        finally:
+            os.environ.pop("SC3D_TEST_GREEN_FORK", None)
+        out.append((by_id(p), m[np.argsort(p[6], kind="stable")]))
+    (pa, ma), (pb, mb) = out
+    assert np.array_equal(ma, mb)
+    assert np.array_equal(pa[6], pb[6])
+    for c in range(6):
+        scale = np.abs(pb[c]).max()
+        err = np.abs(pa[c] - pb[c]).max() / scale
+        print(f"  column {c}: max rel {err:.2e}", flush=True)
+        assert err <= 1e-10, (c, err)
+
+
 def test_multi_step_parity(gpu):
     """Three kicks in a row (sort every 2nd apply) vs three oracle kicks."""
     n = 30000
