@@ -1308,7 +1308,9 @@ __global__ __launch_bounds__(kTileThreads) void k_deposit_tile(
 // Peak of the unit-weight FP32 density (all entries >= 0, so the IEEE bit
 // patterns order like the values): block max (wavefront butterflies, then
 // LDS), one integer atomicMax per block into the domain record (cleared by
-// k_domain_finish each apply); the grid is bounded so few atomics meet.
+// k_domain_finish each apply); the grid is bounded so few atomics meet
+// (measured at 128x128x256: 8.9 us per launch over 512 blocks, 14.8 us with
+// 16-byte loads over 1024 blocks).
 __global__ __launch_bounds__(kThreads) void k_rho_peak(const float* __restrict__ rho, int64_t G,
                                                        DomainRec* __restrict__ dom)
 {
