@@ -1066,17 +1066,30 @@ constexpr int kTileNodes = 4096;                        // FP64 nodes of the LDS
 constexpr int kTileTailDiv = 8;                         // 1/8 of the windows rotated first
 constexpr int kTileWaves = kTileThreads / 64;
 
+// Wavefront min / max, the same value in every lane: prefix reductions
+// inside each row of 16 lanes by DPP moves (a lane with no source keeps its
+// own value), then the four row results read from lanes 15, 31, 47, 63.
+// Every lane of the wavefront calls it.  (Measured against six ds_bpermute
+// exchanges per value, brick-ordered keys, sort period 24, deposit / kick
+// phase: 54.3-54.7 / 64.4-64.7 -> 51.9-52.2 / 61.8-62.1 us at 4e6 FP64,
+// 200.6-200.9 / 237.4-238.3 -> 190.2-190.8 / 233.5-233.8 us at 1.6e7 FP32.)
 __device__ __forceinline__ int wave_min(int v)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o *= 2) v = min(v, __shfl_xor(v, o, 64));
-    return v;
+    v = min(v, dpp_move<kDppRowShr + 1>(v, v));
+    v = min(v, dpp_move<kDppRowShr + 2>(v, v));
+    v = min(v, dpp_move<kDppRowShr + 4>(v, v));
+    v = min(v, dpp_move<kDppRowShr + 8>(v, v));
+    return min(min(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+               min(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
 }
 __device__ __forceinline__ int wave_max(int v)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o *= 2) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+    v = max(v, dpp_move<kDppRowShr + 1>(v, v));
+    v = max(v, dpp_move<kDppRowShr + 2>(v, v));
+    v = max(v, dpp_move<kDppRowShr + 4>(v, v));
+    v = max(v, dpp_move<kDppRowShr + 8>(v, v));
+    return max(max(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+               max(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
 }
 
 // Corner weights of one particle in the reference's order, w0*wx*wy*wz left
